@@ -1,0 +1,102 @@
+// Device-side state and parameters of the SORD solver step.
+//
+// Layout in HBM: every volume field is a 3-D array over the padded local
+// mesh nm = nn + 2*nhalo (sord/setup.f90:56-57), j fastest, stored with a row
+// pitch P0 (a multiple of 128 bytes) and a base offset chosen so that the first
+// interior node of every row starts on a 128-byte boundary.  Vectors and
+// tensors are structure-of-arrays: one such 3-D array per component.
+// Indices used in kernels are the reference's 1-based local indices.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SORD_MAXPT 64   // sub-cell / single-cell ops applied inside a kernel per step
+
+template <class T>
+struct Grid {
+    int nm[3];      // padded local shape
+    int nn[3];      // global shape
+    int nnoff[3];   // local -> global offset
+    int P0;         // row pitch (elements)
+    long P1;        // plane pitch (elements) = P0 * nm[1]
+    long off;       // base offset (elements)
+    long S;         // allocation size of one 3-D array (elements)
+    __host__ __device__ __forceinline__ long at(int j, int k, int l) const {
+        return off + (long)(j - 1) + (long)P0 * (k - 1) + P1 * (long)(l - 1);
+    }
+};
+
+// A small list of per-cell or per-node operations (value already scaled by the
+// time function) applied inside the stress kernel: potency, moment, stress.
+template <class T>
+struct PointOps {
+    int n;
+    int stage[SORD_MAXPT];   // 0: strain (potency p**), 1: moment m**, 2: stress w**
+    int comp[SORD_MAXPT];    // 0..5: xx yy zz yz zx xy
+    int op[SORD_MAXPT];      // 0 '=', 1 '+', 2 '*'
+    long cell[SORD_MAXPT];   // linear index
+    T val[SORD_MAXPT];
+    int lo[3], hi[3];        // bounding box (1-based local cell indices)
+};
+
+template <class T>
+struct Params {
+    Grid<T> g;
+    T dt;
+    // node vectors (current state) and next-state buffers
+    T *u[3], *v[3], *un[3], *vn[3];
+    T *a[3];            // force / acceleration (reference w1 during step_accel)
+    T *sig[6];          // cell stress: xx yy zz (reference w1), yz zx xy (reference w2)
+    T *gam, *mr;        // node: dt * node-averaged viscosity, inverse nodal mass
+    T *lam, *mu, *yy, *vc;  // cell: Lame moduli, hourglass constant, 1/volume
+    T *hx, *hy, *hz;    // cell widths along each axis (rectilinear mesh), 1-based cell index
+    // region bounds (1-based local)
+    int i1node[3], i2node[3], i1cell[3], i2cell[3], i1pml[3], i2pml[3];
+    int i1bc[3], i2bc[3];
+    int bc1[3], bc2[3];
+    // PML
+    int npml;
+    T *dn1, *dn2, *dc1, *dc2;   // damping tables, index p-1
+    T *p[6], *gm[6];            // node / cell memory slabs: lower x,y,z then upper x,y,z
+    int pn[6][3], gn[6][3];     // their shapes (Fortran order, per component)
+    // hourglass
+    int hg_on, hg_pml;
+    T h1, dth2;
+    // fault
+    int ifn, irup;              // 1-based axis (0 none) and local index of the minus side
+    int store_a;                // fused update also stores the acceleration (statistics steps)
+};
+
+struct Box {
+    int lo[3], hi[3], st[3], n[3];  // 1-based local indices, steps, counts
+};
+
+// A strided 3-D view: element (j,k,l) (1-based) is at off + (j-1) + s1*(k-1) + s2*(l-1).
+struct View {
+    long off, s1, s2;
+    __host__ __device__ __forceinline__ long at(int j, int k, int l) const {
+        return off + (long)(j - 1) + s1 * (long)(k - 1) + s2 * (long)(l - 1);
+    }
+};
+
+template <class T>
+struct Samples {
+    int n;
+    int lo[SORD_MAXPT][3], hi[SORD_MAXPT][3];
+    T xi[SORD_MAXPT][3];
+    int slot[SORD_MAXPT];  // output offset
+};
+
+// Fault-plane arrays: the local mesh with extent 1 along the normal, Fortran
+// order (sord/dynamic_rupture.f90:22-63).
+template <class T>
+struct Fault {
+    int fm[3];
+    T *area, *nhat[3], *t0[3];
+    T *mus, *mud, *dc, *co, *rhypo;
+    T *sl, *psv, *trup, *tarr, *svold;
+    T *t1[3], *t3[3], *ts, *tn, *fr, *sa[3], *sam;
+    T *sv[3], *su[3], *svm, *sum;
+    int faultopening, nucleate;
+    T trelax, vrup, rcrit;
+};

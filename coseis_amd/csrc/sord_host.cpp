@@ -1,0 +1,848 @@
+// Host runtime of the MI355X SORD solver: the C ABI declared in
+// include/sord_amd.h.
+//
+// It replaces the reference program sord.x (sord/sord.f90): it parses the
+// sord.in text, plans the local mesh, computes the initial state on the host
+// in the solver's precision (grid, material, PML tables, fault), uploads it to
+// HBM, and drives the HIP kernels of the time loop together with the field
+// i/o, statistics and profile files the reference writes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sord_amd.h"
+#include "sord_launch.h"
+
+namespace sord {
+
+static thread_local std::string g_err;
+
+struct Err {
+    int code;
+    std::string msg;
+};
+[[noreturn]] static void fail(int code, const std::string &msg) { throw Err{code, msg}; }
+
+#define HIPCHECK(x)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) fail(SORD_EHIP, std::string("HIP error ") + hipGetErrorString(e_) + " in " #x); \
+    } while (0)
+
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+static int fmodulo(int a, int p) {
+    int r = a % p;
+    return r < 0 ? r + p : r;
+}
+
+// Host 3-D array, 1-based indices, Fortran order.
+template <class T>
+struct HArr {
+    std::vector<T> d;
+    int n[3] = {0, 0, 0};
+    void alloc(int a, int b, int c, T v = T(0)) {
+        n[0] = a, n[1] = b, n[2] = c;
+        d.assign((size_t)a * b * c, v);
+    }
+    T &operator()(int j, int k, int l) {
+        return d[(size_t)(j - 1) + (size_t)n[0] * ((size_t)(k - 1) + (size_t)n[1] * (size_t)(l - 1))];
+    }
+    size_t size() const { return d.size(); }
+};
+
+// set f outside [i1, i2] to r; singleton axes are left alone (sord/utilities.f90:103-118)
+template <class T>
+static void set_halo(HArr<T> &f, T r, const int i1[3], const int i2[3]) {
+    int i3[3], i4[3];
+    for (int i = 0; i < 3; i++) i3[i] = imin(i1[i], f.n[i] + 1), i4[i] = imax(i2[i], 0);
+    for (int l = 1; l <= f.n[2]; l++)
+        for (int k = 1; k <= f.n[1]; k++)
+            for (int j = 1; j <= f.n[0]; j++)
+                if ((f.n[0] > 1 && (j < i3[0] || j > i4[0])) || (f.n[1] > 1 && (k < i3[1] || k > i4[1])) ||
+                    (f.n[2] > 1 && (l < i3[2] || l > i4[2])))
+                    f(j, k, l) = r;
+}
+
+// scalar_bc plane copies (sord/boundary_cond.f90:7-27)
+template <class T>
+static void scalar_bc(HArr<T> &f, const int bc1[3], const int bc2[3], const int i1[3], const int i2[3]) {
+    for (int a = 0; a < 3; a++) {
+        int b1 = std::abs(bc1[a] % 10), b2 = std::abs(bc2[a] % 10);
+        if (b1 == 4) b1 = 2;
+        if (b2 == 4) b2 = 2;
+        if (f.n[a] == 1 || i1[a] < 1 || i1[a] >= f.n[a]) b1 = 99;
+        if (f.n[a] == 1 || i2[a] <= 1 || i2[a] > f.n[a]) b2 = 99;
+        for (int side = 0; side < 2; side++) {
+            if ((side == 0 ? b1 : b2) != 2) continue;
+            int dst = side == 0 ? i1[a] : i2[a], src = side == 0 ? i1[a] + 1 : i2[a] - 1;
+            int lo[3] = {1, 1, 1}, hi[3] = {f.n[0], f.n[1], f.n[2]};
+            lo[a] = hi[a] = dst;
+            for (int l = lo[2]; l <= hi[2]; l++)
+                for (int k = lo[1]; k <= hi[1]; k++)
+                    for (int j = lo[0]; j <= hi[0]; j++) {
+                        int s[3] = {j, k, l};
+                        s[a] = src;
+                        f(j, k, l) = f(s[0], s[1], s[2]);
+                    }
+        }
+    }
+}
+
+// Pulse time functions (sord/utilities.f90:223-282); pi is the reference's
+// literal 3.14159265 in the solver precision.
+template <class T>
+static T time_function(const std::string &pulse, T t, T dt, T tau) {
+    const T pi = T(3.14159265);
+    T f = 0, a, b;
+    if (pulse == "const") f = 1;
+    else if (pulse == "delta") {
+        if (std::fabs(t) < T(0.25) * dt) f = T(1) / dt;
+    } else if (pulse == "step" || pulse == "integral_delta") {
+        if (std::fabs(t) < T(0.25) * dt) f = T(0.5);
+        else if (t >= T(0.25) * dt) f = 1;
+    } else if (pulse == "brune") {
+        if (0 < t) a = T(1) / tau, f = std::exp(-a * t) * a * a * t;
+    } else if (pulse == "integral_brune") {
+        if (0 < t) a = T(1) / tau, f = T(1) - std::exp(-a * t) * (a * t + T(1));
+    } else if (pulse == "hann") {
+        b = pi * tau;
+        if (-b < t && t < b) a = T(1) / tau, f = T(0.5) / pi * a * (T(1) + std::cos(a * t));
+    } else if (pulse == "integral_hann") {
+        b = pi * tau;
+        if (-b < t && t < b) a = T(1) / tau, f = T(0.5) + T(0.5) / pi * (a * t + std::sin(a * t));
+        else if (0 < t) f = 1;
+    } else if (pulse == "gaussian" || pulse == "integral_ricker1") {
+        a = T(0.5) / (tau * tau);
+        b = std::sqrt(a / pi);
+        f = std::exp(-a * t * t) * b;
+    } else if (pulse == "ricker1" || pulse == "integral_ricker2") {
+        a = T(0.5) / (tau * tau);
+        b = std::sqrt(a / pi) * T(2) * a;
+        f = -std::exp(-a * t * t) * b * t;
+    } else if (pulse == "ricker2") {
+        a = T(0.5) / (tau * tau);
+        b = std::sqrt(a / pi) * T(4) * a;
+        f = std::exp(-a * t * t) * b * (a * t * t - T(0.5));
+    } else {
+        fail(SORD_EBADOP, "invalid time func: " + pulse);
+    }
+    return f;
+}
+
+// One field i/o record (sord/input_output.f90:4-12)
+template <class T>
+struct IoRec {
+    std::string field, fname;
+    char reg = 'n';
+    char op[3] = {0, 0, 0};
+    int ii[4][3];  // per dimension: start (1-based), stop, step
+    int nb = 1;
+    T x1[3], x2[3], val, tau;
+    // runtime state
+    int ib = -1;
+    std::vector<T> hbuf;   // host buffer (nb records)
+    T *dbuf = nullptr;     // device staging (nb records) for device outputs
+    size_t rec = 0;
+};
+
+// list-directed tokens of one line
+static std::vector<std::string> tokens(const std::string &line) {
+    std::vector<std::string> out;
+    size_t i = 0, n = line.size();
+    while (i < n) {
+        while (i < n && (line[i] == ' ' || line[i] == '\t' || line[i] == ',' || line[i] == '\r')) i++;
+        if (i >= n) break;
+        size_t j = i;
+        while (j < n && line[j] != ' ' && line[j] != '\t' && line[j] != ',' && line[j] != '\r') j++;
+        out.push_back(line.substr(i, j - i));
+        i = j;
+    }
+    return out;
+}
+
+struct TokReader {
+    std::vector<std::string> t;
+    size_t p = 0;
+    const std::string &next() {
+        if (p >= t.size()) fail(SORD_EINPUT, "sord.in: too few values");
+        return t[p++];
+    }
+    int i() {
+        const std::string &s = next();
+        char *e;
+        long v = strtol(s.c_str(), &e, 10);
+        if (*e) fail(SORD_EINPUT, "sord.in: bad integer '" + s + "'");
+        return (int)v;
+    }
+    template <class T>
+    T r() {
+        const std::string &s = next();
+        char *e;
+        T v = sizeof(T) == 4 ? (T)strtof(s.c_str(), &e) : (T)strtod(s.c_str(), &e);
+        if (*e) fail(SORD_EINPUT, "sord.in: bad real '" + s + "'");
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Solver state
+// ---------------------------------------------------------------------------
+template <class T>
+struct Solver {
+    // input parameters (sord/globals.f90:5-16)
+    int bc1[3], bc2[3], debug, faultopening, i1pml[3], i2pml[3], itio, itstats, mpin, mpout, n1expand[3],
+        n2expand[3], npml, nproc3[3], nsource, nthread, ppml, shape[4];
+    T affine[9], delta[4], gam_max, gam_min, gridnoise, hourglass[2], hypocenter[3], rcrit, rexpand, rho_max,
+        rho_min, slipvector[3], svtol, tm0, trelax, vdamp, vp_max, vp_min, vpml, vrup, vs_max, vs_min;
+    std::string source, faultnormal;
+    std::vector<IoRec<T>> io;
+    // derived (sord/globals.f90:18-40, sord/setup.f90)
+    T dt, dx[3], tm = 0;
+    int it = 0, nt = 0, ifn = 0, irup = 0;
+    int nn[3], nm[3], nhalo[3], nnoff[3], i1bc[3], i2bc[3], i1core[3], i2core[3], i1node[3], i2node[3],
+        i1cell[3], i2cell[3];
+    int fm[3];
+    // host initial state
+    HArr<T> xx[3], xc[3], vc, mr, lam, mu, gam, yy, s1, s2;
+    std::vector<T> wax[3];  // cell widths per axis (1-based cell index, size nm+1)
+    std::vector<T> dn1, dn2, dc1, dc2;
+    // fault (host init)
+    HArr<T> mus, mud, dcf, co, area, rhypo, lamf, muf, fnh[3], ft0[3], ft1[3], ft2[3], ft3[3];
+    // statistics (sord/statistics.f90:3-9)
+    T amax = 0, vmax = 0, umax = 0, wmax = 0, samax = 0, svmax = 0, sumax = 0, slmax = 0, tsmax = 0, tnmin = 0,
+      tnmax = 0, tarrmax = 0, efric = 0, estrain = 0, moment = 0;
+    int jstat = 0;
+    std::vector<T> maxl, suml;
+    std::vector<int32_t> prof;
+    int64_t timers[4] = {0, 0, 0, 0};
+    std::chrono::steady_clock::time_point clock0;
+    // device
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> dallocs;
+    Params<T> P;
+    Fault<T> F;
+    View vview;          // volume arrays
+    View fview;          // fault planes
+    T *ub[2][3], *vb[2][3];
+    int cur = 0;
+    bool time_pending = false;  // the leapfrog of the next step is still to do (split mode)
+    bool have_sig = false;
+    T *dscratch = nullptr;  // one volume array of scratch
+    T *dlam2 = nullptr, *dmu2 = nullptr;  // fused path: lam/V and 2 mu/V
+    T *red_val = nullptr;
+    long *red_loc = nullptr;
+    bool use_fused = true;
+    bool initialized = false;
+    int state_bc_local = 1;  // all boundary conditions plane-local (-1, 1, 3, 0, pml)
+
+    int64_t clock_ms() const {
+        return std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - clock0)
+            .count();
+    }
+
+    ~Solver() {
+        for (void *p : dallocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    T *dalloc(size_t n) {
+        void *p = nullptr;
+        HIPCHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+        HIPCHECK(hipMemsetAsync(p, 0, (n ? n : 1) * sizeof(T), stream));
+        dallocs.push_back(p);
+        return (T *)p;
+    }
+
+    // ---------------- read_parameters: sord/input_output.f90:21-98 ----------------
+    void parse(const char *text) {
+        std::vector<std::string> lines;
+        {
+            std::string s(text);
+            size_t i = 0;
+            while (i <= s.size()) {
+                size_t j = s.find('\n', i);
+                if (j == std::string::npos) j = s.size();
+                lines.push_back(s.substr(i, j - i));
+                i = j + 1;
+            }
+        }
+        if (lines.empty()) fail(SORD_EINPUT, "sord.in: empty");
+        TokReader r;
+        r.t = tokens(lines[0]);
+        for (int i = 0; i < 9; i++) affine[i] = r.r<T>();
+        for (int i = 0; i < 3; i++) bc1[i] = r.i();
+        for (int i = 0; i < 3; i++) bc2[i] = r.i();
+        debug = r.i();
+        for (int i = 0; i < 4; i++) delta[i] = r.r<T>();
+        faultnormal = r.next().substr(0, 2);
+        faultopening = r.i();
+        gam_max = r.r<T>();
+        gam_min = r.r<T>();
+        gridnoise = r.r<T>();
+        for (int i = 0; i < 2; i++) hourglass[i] = r.r<T>();
+        for (int i = 0; i < 3; i++) hypocenter[i] = r.r<T>();
+        for (int i = 0; i < 3; i++) i1pml[i] = r.i();
+        for (int i = 0; i < 3; i++) i2pml[i] = r.i();
+        itio = r.i();
+        itstats = r.i();
+        mpin = r.i();
+        mpout = r.i();
+        for (int i = 0; i < 3; i++) n1expand[i] = r.i();
+        for (int i = 0; i < 3; i++) n2expand[i] = r.i();
+        int nfieldio = r.i();
+        npml = r.i();
+        for (int i = 0; i < 3; i++) nproc3[i] = r.i();
+        nsource = r.i();
+        nthread = r.i();
+        ppml = r.i();
+        rcrit = r.r<T>();
+        rexpand = r.r<T>();
+        rho_max = r.r<T>();
+        rho_min = r.r<T>();
+        for (int i = 0; i < 4; i++) shape[i] = r.i();
+        for (int i = 0; i < 3; i++) slipvector[i] = r.r<T>();
+        source = r.next().substr(0, 16);
+        svtol = r.r<T>();
+        tm0 = r.r<T>();
+        trelax = r.r<T>();
+        vdamp = r.r<T>();
+        vp_max = r.r<T>();
+        vp_min = r.r<T>();
+        vpml = r.r<T>();
+        vrup = r.r<T>();
+        vs_max = r.r<T>();
+        vs_min = r.r<T>();
+        for (int i = 0; i < 3; i++) i1pml[i] += 1;  // sord/input_output.f90:62
+        if (nfieldio < 0 || (size_t)nfieldio + 1 > lines.size())
+            fail(SORD_EINPUT, "sord.in: too few field i/o records");
+        io.resize(nfieldio);
+        for (int q = 0; q < nfieldio; q++) {
+            TokReader t;
+            t.t = tokens(lines[q + 1]);
+            IoRec<T> &o = io[q];
+            o.field = t.next().substr(0, 4);
+            o.reg = t.next()[0];
+            for (int d = 0; d < 4; d++)
+                for (int e = 0; e < 3; e++) o.ii[d][e] = t.i();
+            o.nb = t.i();
+            for (int i = 0; i < 3; i++) o.x1[i] = t.r<T>();
+            for (int i = 0; i < 3; i++) o.x2[i] = t.r<T>();
+            o.val = t.r<T>();
+            o.tau = t.r<T>();
+            std::string op = t.next();
+            o.op[0] = op[0];
+            o.op[1] = op.size() > 1 ? op[1] : ' ';
+            o.fname = t.next().substr(0, 32);
+            for (int d = 0; d < 4; d++) o.ii[d][0] += 1;  // sord/input_output.f90:84-86
+            if (o.nb < 1) o.nb = 1;
+            for (int d = 0; d < 4; d++)
+                if (o.ii[d][2] < 1) fail(SORD_EINPUT, "sord.in: i/o step must be positive");
+        }
+    }
+
+    // ---------------- setup_dimensions: sord/setup.f90:6-119 (one process) ----------------
+    void setup() {
+        for (int i = 0; i < 3; i++) dx[i] = delta[i];
+        dt = delta[3];
+        for (int i = 0; i < 3; i++) nn[i] = shape[i];
+        nt = imax(shape[3], 0);
+        if (faultnormal.size() < 2) fail(SORD_EINPUT, "Error in faultnormal");
+        switch (faultnormal[1]) {
+        case 'o': ifn = 0; break;
+        case 'x': ifn = 1; break;
+        case 'y': ifn = 2; break;
+        case 'z': ifn = 3; break;
+        default: fail(SORD_EINPUT, "Error in faultnormal");
+        }
+        for (int i = 0; i < 3; i++) {
+            if (nproc3[i] != 1) fail(SORD_EUNSUPPORTED, "one GPU per run: nproc3 must be 1 1 1");
+            if (nn[i] < 2) fail(SORD_EINPUT, "shape: every axis needs at least 2 nodes");
+            nhalo[i] = 1;
+        }
+        if (ifn) nhalo[ifn - 1] = 2;
+        for (int i = 0; i < 3; i++) {
+            nnoff[i] = -nhalo[i];
+            nm[i] = nn[i] + 2 * nhalo[i];
+            i1bc[i] = 1 - nnoff[i];
+            i2bc[i] = nn[i] - nnoff[i];
+            i1core[i] = 1 + nhalo[i];
+            i2core[i] = nm[i] - nhalo[i];
+            i1node[i] = imax(i1bc[i], 2);
+            i2node[i] = imin(i2bc[i], nm[i] - 1);
+            i1cell[i] = imax(i1bc[i], 1);
+            i2cell[i] = imin(i2bc[i] - 1, nm[i] - 1);
+            i1pml[i] -= nnoff[i];
+            i2pml[i] -= nnoff[i];
+        }
+        irup = 0;
+        if (ifn) {
+            irup = (int)std::floor((double)(hypocenter[ifn - 1] + T(0.000001))) - nnoff[ifn - 1] + 1;
+            if (irup + 1 < i1core[ifn - 1] || irup > i2core[ifn - 1]) ifn = 0;
+        }
+        // PML tables hold npml entries: a damped plane beyond them would read
+        // outside the reference's arrays (sord/boundary_pml.f90:136-150)
+        if (npml > 0) {
+            for (int a = 0; a < 3; a++) {
+                int lo_nodes = imin(i2node[a], i1pml[a]) - i1node[a] + 1;
+                int hi_nodes = i2node[a] - imax(i1node[a], i2pml[a]) + 1;
+                if (lo_nodes > npml || hi_nodes > npml)
+                    fail(SORD_EINPUT, "PML box deeper than npml planes: the reference reads its damping tables "
+                                      "out of bounds for this i1pml/i2pml (sord/acceleration.f90:300-306); "
+                                      "stage with coseis_amd.sord");
+                if (i1pml[a] + 1 > i2pml[a] - 1) fail(SORD_EINPUT, "Error: model too small for PML");
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            int t1 = bc1[a] % 10, t2 = bc2[a] % 10;
+            if (!(t1 == 0 || t1 == 1 || t1 == -1 || t1 == 3)) state_bc_local = 0;
+            if (!(t2 == 0 || t2 == 1 || t2 == -1 || t2 == 3)) state_bc_local = 0;
+        }
+        if (itio < 1 || itstats < 1) fail(SORD_EINPUT, "itio and itstats must be positive");
+        if (nsource != 0) fail(SORD_EUNSUPPORTED, "finite kinematic source (nsource != 0)");
+        if (gridnoise > 0) fail(SORD_EUNSUPPORTED, "gridnoise (values of the Fortran random number generator)");
+        for (int a = 0; a < 3; a++) fm[a] = nm[a];
+        if (ifn) fm[ifn - 1] = 1;
+    }
+
+    // ---------------- host field i/o (static fields at it = 0) ----------------
+    // A host target is an HArr; f.n gives its extents (1 along the fault normal
+    // for fault arrays).  Semantics of sord/input_output.f90:103-542.
+    struct Index {
+        int i1[3], i2[3], di[3], m[4], n[4], o[4];
+        T xi[3];
+    };
+
+    // spatial indices of record o for the current pass; returns false to skip
+    bool io_match(IoRec<T> &o, char pass, const std::string &field) {
+        const int it1 = o.ii[3][0], it2 = o.ii[3][1], dit = o.ii[3][2];
+        if (it > it2) return false;
+        if (o.op[0] == '#') return false;
+        if (it < it1) return false;
+        if (fmodulo(it - it1, dit) != 0) return false;
+        if (field != o.field) return false;
+        if (pass == '<' && o.op[1] == '>') return false;
+        if (pass == '>' && o.op[1] != '>') return false;
+        return true;
+    }
+
+    // non-disk index box (sord/input_output.f90:153-173)
+    void io_box_nondisk(IoRec<T> &o, Index &x) {
+        int i3[3], i4[3];
+        for (int i = 0; i < 3; i++) {
+            if (o.reg == 'n') {
+                i3[i] = i1node[i], i4[i] = i2node[i];
+                x.xi[i] = o.x1[i] - (T)nnoff[i] + T(1);
+            } else {
+                i3[i] = i1cell[i], i4[i] = i2cell[i];
+                x.xi[i] = o.x1[i] - (T)nnoff[i] + T(0.5);
+            }
+        }
+        if (o.op[0] == '.') {
+            for (int i = 0; i < 3; i++) {
+                int fl = (int)std::floor((double)x.xi[i]);
+                x.i1[i] = imax(i3[i], fl);
+                x.i2[i] = imin(i4[i], fl + 1);
+                x.di[i] = 1;
+            }
+        } else {
+            for (int i = 0; i < 3; i++) {
+                x.i1[i] = o.ii[i][0] - nnoff[i];
+                x.i2[i] = o.ii[i][1] - nnoff[i];
+                x.di[i] = o.ii[i][2];
+                if (x.i1[i] < i3[i]) x.i1[i] += ((i3[i] - x.i1[i] - 1) / x.di[i] + 1) * x.di[i];
+                if (x.i2[i] > i4[i]) x.i2[i] = x.i1[i] + (i4[i] - x.i1[i]) / x.di[i] * x.di[i];
+            }
+        }
+    }
+
+    // disk index box (sord/input_output.f90:261-293); returns false if disabled
+    bool io_box_disk(IoRec<T> &o, Index &x, const int fn[3]) {
+        if (o.op[0] == '.') {
+            for (int i = 0; i < 4; i++) x.m[i] = x.n[i] = 1, x.o[i] = 0;
+        } else {
+            for (int i = 0; i < 3; i++) {
+                x.i1[i] = o.ii[i][0] - nnoff[i];
+                x.i2[i] = o.ii[i][1] - nnoff[i];
+                x.di[i] = o.ii[i][2];
+                int i3 = x.i1[i], i4 = x.i2[i];
+                if (x.i1[i] < i1core[i]) x.i1[i] += ((i1core[i] - x.i1[i] - 1) / x.di[i] + 1) * x.di[i];
+                if (x.i2[i] > i2core[i]) x.i2[i] = x.i1[i] + (i2core[i] - x.i1[i]) / x.di[i] * x.di[i];
+                x.m[i] = (i4 - i3) / x.di[i] + 1;
+                x.n[i] = imax((x.i2[i] - x.i1[i]) / x.di[i] + 1, 0);
+                x.o[i] = (x.i1[i] - i3) / x.di[i];
+            }
+            x.n[3] = 0;
+        }
+        for (int i = 0; i < 3; i++) {
+            if (fn[i] == 1) {
+                if (x.n[i] < 1) {
+                    o.op[0] = '#';
+                    return false;
+                }
+                x.i1[i] = x.i2[i] = 1;
+                x.m[i] = x.n[i] = 1;
+                x.o[i] = 0;
+            }
+        }
+        return true;
+    }
+
+    void file_write(const std::string &fname, const T *buf, size_t reclen, int o, int n) {
+        FILE *fp = fopen(fname.c_str(), o > 0 ? "r+b" : "wb");
+        if (!fp) fail(SORD_EFILE, "cannot write " + fname);
+        fseek(fp, (long)((size_t)o * reclen * sizeof(T)), SEEK_SET);
+        size_t w = fwrite(buf, sizeof(T), reclen * (size_t)n, fp);
+        fclose(fp);
+        if (w != reclen * (size_t)n) fail(SORD_EFILE, "short write " + fname);
+    }
+    void file_read(const std::string &fname, T *buf, size_t reclen, int o, int n) {
+        FILE *fp = fopen(fname.c_str(), "rb");
+        if (!fp) fail(SORD_EFILE, "cannot open input file " + fname);
+        fseek(fp, (long)((size_t)o * reclen * sizeof(T)), SEEK_SET);
+        size_t r = fread(buf, sizeof(T), reclen * (size_t)n, fp);
+        fclose(fp);
+        if (r != reclen * (size_t)n) fail(SORD_EFILE, "short read " + fname);
+        for (size_t q = 0; q < reclen * (size_t)n; q++)
+            if (!std::isfinite((double)buf[q])) fail(SORD_ENANIN, "NaN/Inf in " + fname);
+    }
+
+    // disk input: values over the final box after the singleton broadcast
+    // (sord/input_output.f90:295-437).  Fills lo/hi and vals (Fortran order).
+    bool io_read(IoRec<T> &o, Index &x, const int fn[3], int lo[3], int hi[3], std::vector<T> &vals) {
+        const int it1 = o.ii[3][0], it2 = o.ii[3][1], dit = o.ii[3][2];
+        if (o.op[0] == '.') fail(SORD_EUNSUPPORTED, "'.<' (the reference reads a stale buffer row, "
+                                                    "sord/input_output.f90:350)");
+        for (int i = 0; i < 3; i++)
+            if (x.m[i] == 1) x.i1[i] = x.i2[i] = 1, x.n[i] = 1, x.o[i] = 0;
+        for (int i = 0; i < 3; i++)
+            if (x.di[i] > 1) fail(SORD_EUNSUPPORTED, "strided disk input (interpolate)");
+        size_t rec = (size_t)x.n[0] * x.n[1] * x.n[2];
+        if (o.ib < 0) {
+            o.hbuf.assign(rec * o.nb, T(0));
+            o.ib = o.nb;
+        }
+        if (o.ib == o.nb) {
+            x.n[3] = imin(o.nb, (it2 - it) / dit + 1);
+            x.o[3] = (it - it1) / dit;
+            if (x.n[0] < 1 || x.n[1] < 1 || x.n[2] < 1 || x.n[3] < 1) {
+                o.hbuf.clear();
+                o.op[0] = '#';
+                return false;
+            }
+            file_read(o.fname, o.hbuf.data(), rec, x.o[3], x.n[3]);
+            o.ib = 0;
+        }
+        o.ib++;
+        const T *b = o.hbuf.data() + rec * (size_t)(o.ib - 1);
+        for (int i = 0; i < 3; i++) {
+            lo[i] = x.i1[i];
+            hi[i] = x.m[i] == 1 ? fn[i] : x.i2[i];
+        }
+        vals.resize((size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1));
+        size_t c = 0;
+        for (int l = lo[2]; l <= hi[2]; l++)
+            for (int k = lo[1]; k <= hi[1]; k++)
+                for (int j = lo[0]; j <= hi[0]; j++) {
+                    int p[3] = {j - x.i1[0], k - x.i1[1], l - x.i1[2]};
+                    for (int i = 0; i < 3; i++)
+                        if (x.m[i] == 1) p[i] = 0;
+                    vals[c++] = b[(size_t)p[0] + (size_t)x.n[0] * ((size_t)p[1] + (size_t)x.n[1] * p[2])];
+                }
+        if (it == it2) {
+            o.hbuf.clear();
+            o.op[0] = '#';
+        }
+        return true;
+    }
+
+    // buffered output flush (sord/input_output.f90:494-508)
+    void io_flush(IoRec<T> &o, Index &x, bool force_close) {
+        const int it1 = o.ii[3][0], it2 = o.ii[3][1], dit = o.ii[3][2];
+        if (o.ib == o.nb || it == it2 || fmodulo(it, itio) == 0 || force_close) {
+            x.n[3] = o.ib;
+            x.o[3] = (it - it1) / dit + 1 - x.n[3];
+            if (o.dbuf) {
+                o.hbuf.resize(o.rec * o.nb);
+                HIPCHECK(hipMemcpyAsync(o.hbuf.data(), o.dbuf, o.rec * o.ib * sizeof(T), hipMemcpyDeviceToHost,
+                                        stream));
+                HIPCHECK(hipStreamSynchronize(stream));
+            }
+            if (o.rec >= 1 && x.n[3] >= 1) file_write(o.fname, o.hbuf.data(), o.rec, x.o[3], x.n[3]);
+            o.ib = 0;
+            if (it == it2 || o.rec < 1) {
+                o.hbuf.clear();
+                o.op[0] = '#';
+            }
+        }
+    }
+
+    void host_io(const char *passes, const std::string &field, HArr<T> &f) {
+        int64_t t0c = clock_ms();
+        const int one[3] = {1, 1, 1};
+        (void)one;
+        for (const char *pp = passes; *pp; pp++) {
+            for (auto &o : io) {
+                if (!io_match(o, *pp, field)) continue;
+                Index x;
+                io_box_nondisk(o, x);
+                if (o.op[1] != '<' && o.op[1] != '>') {
+                    if (x.i2[0] < x.i1[0] || x.i2[1] < x.i1[1] || x.i2[2] < x.i1[2]) {
+                        o.op[0] = '#';
+                        continue;
+                    }
+                    T val = o.val * time_function<T>(o.fname, tm, dt, o.tau);
+                    if (o.reg == 'f')  // defect D14: the intended fault plane
+                        for (int i = 0; i < 3; i++)
+                            if (f.n[i] == 1) x.i1[i] = x.i2[i] = 1;
+                    std::string op(o.op);
+                    if (op == ". ") {
+                        for (int l = x.i1[2]; l <= x.i2[2]; l++)
+                            for (int k = x.i1[1]; k <= x.i2[1]; k++)
+                                for (int j = x.i1[0]; j <= x.i2[0]; j++) {
+                                    T r = (T(1) - std::fabs(x.xi[0] - (T)j)) * (T(1) - std::fabs(x.xi[1] - (T)k)) *
+                                          (T(1) - std::fabs(x.xi[2] - (T)l));
+                                    f(j, k, l) = f(j, k, l) + val * r;
+                                }
+                    } else if (op == "= " || op == "+ " || op == "* ") {
+                        for (int l = x.i1[2]; l <= x.i2[2]; l += x.di[2])
+                            for (int k = x.i1[1]; k <= x.i2[1]; k += x.di[1])
+                                for (int j = x.i1[0]; j <= x.i2[0]; j += x.di[0]) {
+                                    if (op[0] == '=') f(j, k, l) = val;
+                                    else if (op[0] == '+') f(j, k, l) = f(j, k, l) + val;
+                                    else f(j, k, l) = f(j, k, l) * val;
+                                }
+                    } else {
+                        fail(SORD_EUNSUPPORTED, "i/o mode '" + op + "' (random and cube ops)");
+                    }
+                    bcio(o, f);
+                    continue;
+                }
+                if (!io_box_disk(o, x, f.n)) continue;
+                if (o.op[1] == '<') {
+                    int lo[3], hi[3];
+                    std::vector<T> vals;
+                    if (!io_read(o, x, f.n, lo, hi, vals)) continue;
+                    size_t c = 0;
+                    for (int l = lo[2]; l <= hi[2]; l++)
+                        for (int k = lo[1]; k <= hi[1]; k++)
+                            for (int j = lo[0]; j <= hi[0]; j++) {
+                                T v = vals[c++];
+                                if (o.op[0] == '=') f(j, k, l) = v;
+                                else if (o.op[0] == '+') f(j, k, l) = f(j, k, l) + v;
+                                else if (o.op[0] == '*') f(j, k, l) = f(j, k, l) * v;
+                                else fail(SORD_EBADOP, "bad i/o op");
+                            }
+                    bcio(o, f);
+                    continue;
+                }
+                // disk output
+                if (o.op[0] == '.') {
+                    for (int i = 0; i < 3; i++)
+                        if (x.i2[i] < i1core[i] || x.i2[i] > i2core[i]) {
+                            o.op[0] = '#';
+                            break;
+                        }
+                    if (o.op[0] == '#') continue;
+                }
+                o.rec = (size_t)x.n[0] * x.n[1] * x.n[2];
+                if (o.ib < 0) {
+                    o.hbuf.assign(o.rec * o.nb, T(0));
+                    o.ib = 0;
+                }
+                o.ib++;
+                T *b = o.hbuf.data() + o.rec * (size_t)(o.ib - 1);
+                if (o.op[0] == '.') {
+                    b[0] = 0;
+                    for (int l = x.i1[2]; l <= x.i2[2]; l++)
+                        for (int k = x.i1[1]; k <= x.i2[1]; k++)
+                            for (int j = x.i1[0]; j <= x.i2[0]; j++) {
+                                T r = (T(1) - std::fabs(x.xi[0] - (T)j)) * (T(1) - std::fabs(x.xi[1] - (T)k)) *
+                                      (T(1) - std::fabs(x.xi[2] - (T)l));
+                                b[0] = b[0] + f(j, k, l) * r;
+                            }
+                } else {
+                    size_t c = 0;
+                    for (int l = x.i1[2]; l <= x.i2[2]; l += x.di[2])
+                        for (int k = x.i1[1]; k <= x.i2[1]; k += x.di[1])
+                            for (int j = x.i1[0]; j <= x.i2[0]; j += x.di[0]) b[c++] = f(j, k, l);
+                }
+                io_flush(o, x, false);
+            }
+        }
+        timers[1] += clock_ms() - t0c;
+    }
+
+    void bcio(IoRec<T> &o, HArr<T> &f) {
+        if (o.reg == 'n') scalar_bc(f, bc1, bc2, i1bc, i2bc);
+        if (o.reg == 'c') {
+            int t[3] = {i2bc[0] - 1, i2bc[1] - 1, i2bc[2] - 1};
+            scalar_bc(f, bc1, bc2, i1bc, t);
+        }
+    }
+
+#include "sord_init.inc"
+#include "sord_step.inc"
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct Handle {
+    int realsize;
+    std::unique_ptr<Solver<float>> s4;
+    std::unique_ptr<Solver<double>> s8;
+};
+
+template <class F>
+static int guard(F &&f) {
+    try {
+        g_err.clear();
+        f();
+        return SORD_OK;
+    } catch (const Err &e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "out of host memory";
+        return SORD_EHIP;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return SORD_EINPUT;
+    }
+}
+
+template <class Fn>
+static int dispatch(void *h, Fn &&fn) {
+    if (!h) {
+        g_err = "null handle";
+        return SORD_ESTATE;
+    }
+    Handle *H = (Handle *)h;
+    if (H->realsize == 4) return guard([&] { fn(*H->s4); });
+    return guard([&] { fn(*H->s8); });
+}
+
+}  // namespace sord
+
+using namespace sord;
+
+extern "C" int sord_create(const char *text, int realsize, int device, void **handle) {
+    if (!handle || !text) {
+        g_err = "null argument";
+        return SORD_EINPUT;
+    }
+    *handle = nullptr;
+    if (realsize != 4 && realsize != 8) {
+        g_err = "realsize must be 4 or 8";
+        return SORD_EINPUT;
+    }
+    Handle *H = new Handle();
+    H->realsize = realsize;
+    int rc = guard([&] {
+        int n = 0;
+        HIPCHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) fail(SORD_EHIP, "no such HIP device");
+        if (realsize == 4) {
+            H->s4.reset(new Solver<float>());
+            H->s4->device = device;
+            H->s4->parse(text);
+            H->s4->setup();
+        } else {
+            H->s8.reset(new Solver<double>());
+            H->s8->device = device;
+            H->s8->parse(text);
+            H->s8->setup();
+        }
+    });
+    if (rc != SORD_OK) {
+        delete H;
+        return rc;
+    }
+    *handle = H;
+    return SORD_OK;
+}
+
+extern "C" int sord_init(void *h) {
+    return dispatch(h, [](auto &S) {
+        if (S.initialized) fail(SORD_ESTATE, "sord_init called twice");
+        HIPCHECK(hipSetDevice(S.device));
+        S.run_init();
+    });
+}
+
+extern "C" int sord_step(void *h, int nsteps) {
+    return dispatch(h, [nsteps](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_step before sord_init");
+        HIPCHECK(hipSetDevice(S.device));
+        for (int i = 0; i < nsteps && S.it < S.nt; i++) S.step_one();
+        HIPCHECK(hipStreamSynchronize(S.stream));
+    });
+}
+
+extern "C" int sord_run(void *h) {
+    return dispatch(h, [](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_run before sord_init");
+        HIPCHECK(hipSetDevice(S.device));
+        while (S.it < S.nt) S.step_one();
+        HIPCHECK(hipStreamSynchronize(S.stream));
+    });
+}
+
+extern "C" int sord_status(void *h, int *it, int *nt) {
+    return dispatch(h, [&](auto &S) {
+        if (it) *it = S.it;
+        if (nt) *nt = S.nt;
+    });
+}
+
+extern "C" int sord_field(void *h, const char *name, void **ptr, int64_t shape[3], int64_t strides[3],
+                          int64_t *offset) {
+    return dispatch(h, [&](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_field before sord_init");
+        std::string n(name ? name : "");
+        void *p = nullptr;
+        static const char *vn[3] = {"vx", "vy", "vz"}, *un[3] = {"ux", "uy", "uz"};
+        static const char *wn[6] = {"wxx", "wyy", "wzz", "wyz", "wzx", "wxy"};
+        for (int i = 0; i < 3; i++) {
+            if (n == vn[i]) p = S.P.v[i];
+            if (n == un[i]) p = S.P.u[i];
+        }
+        for (int i = 0; i < 6; i++)
+            if (n == wn[i]) p = S.P.sig[i];
+        if (!p) fail(SORD_EINPUT, "no device field named '" + n + "'");
+        HIPCHECK(hipStreamSynchronize(S.stream));
+        *ptr = p;
+        for (int i = 0; i < 3; i++) shape[i] = S.nm[i];
+        strides[0] = 1, strides[1] = S.P.g.P0, strides[2] = S.P.g.P1;
+        *offset = S.P.g.off;
+    });
+}
+
+extern "C" int sord_set_fused(void *h, int on) {
+    return dispatch(h, [on](auto &S) { S.use_fused = on != 0; });
+}
+
+extern "C" int sord_bench(void *h, int nsteps, double *ms_total, double *ms_kernel, char *kname, int name_len) {
+    return dispatch(h, [&](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_bench before sord_init");
+        HIPCHECK(hipSetDevice(S.device));
+        S.bench(nsteps, ms_total, ms_kernel, kname, name_len);
+    });
+}
+
+extern "C" const char *sord_last_error(void) { return g_err.c_str(); }
+
+extern "C" void sord_destroy(void *h) {
+    Handle *H = (Handle *)h;
+    delete H;
+}

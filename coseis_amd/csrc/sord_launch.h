@@ -1,0 +1,23 @@
+// Host-callable launchers of the SORD kernels (explicitly instantiated for
+// float and double in sord_kernels.hip and sord_fused.hip).
+#pragma once
+#include "sord_dev.h"
+
+namespace sord {
+template <class T> void launch_stress(const Params<T> &P, const PointOps<T> &ops, int3 lo, int3 hi, int mode, hipStream_t s);
+template <class T> void launch_accel(const Params<T> &P, int3 lo, int3 hi, int mode, hipStream_t s);
+template <class T> void launch_time(const Params<T> &P, int3 lo, int3 hi, hipStream_t s);
+template <class T> void launch_newton(const Params<T> &P, int3 lo, int3 hi, hipStream_t s);
+template <class T> void launch_gather(const T *f, View g, const int lo[3], const int hi[3], const int st[3], const int n[3], T *out, hipStream_t s);
+template <class T> void launch_boxop(T *f, View g, const int lo[3], const int hi[3], const int st[3], const int n[3], int op, T val, const T *src, hipStream_t s);
+template <class T> void launch_plane_copy(T *f, View g, const int n[3], int ax, int dst, int src, hipStream_t s);
+template <class T> void launch_sample(const T *f, View g, const Samples<T> &smp, T *out, hipStream_t s);
+template <class T> void launch_point_add(T *f, View g, const int lo[3], const int hi[3], const T xi[3], T val, hipStream_t s);
+template <class T> void launch_norm(T *out, T *const a[3], T *const b[3], const Grid<T> &g, const int lo[3], const int hi[3], const int st[3], const int n[3], hipStream_t s);
+template <class T> int launch_maxnorm(T *const a[3], T *const b[3], const Grid<T> &g, const int lo[3], const int hi[3], T *pval, long *ploc, hipStream_t s);
+template <class T> void launch_vbc_plane(const Params<T> &P, int ax, int dst, int src, int mask, const T sg[3], int kind, hipStream_t s);
+template <class T> void launch_rupture(const Params<T> &P, const Fault<T> &F, int it, T tm, int fused, hipStream_t s);
+template <class T> void launch_fault_time(const Params<T> &P, const Fault<T> &F, T tm, T svtol, hipStream_t s);
+template <class T> void launch_fault_svold(const Params<T> &P, const Fault<T> &F, hipStream_t s);
+template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, hipStream_t s);
+}  // namespace sord

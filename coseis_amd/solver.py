@@ -1,0 +1,127 @@
+"""
+ctypes binding of the HIP solver library (C ABI: include/sord_amd.h).
+
+The library is loaded from this package directory; there is no fallback: a
+missing or unloadable library raises ``ImportError``-class errors at first use.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, 'libsord_amd.so')
+
+# error codes of include/sord_amd.h
+ERRORS = {
+    1: 'EINPUT', 2: 'ECFL', 3: 'EMODULI', 4: 'EVOLUME', 5: 'ENAN', 6: 'ENANIN',
+    7: 'EBADOP', 8: 'EFILE', 9: 'EHIP', 10: 'EUNSUPPORTED', 11: 'ESTATE',
+}
+EXPORTS = [
+    'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field',
+    'sord_bench', 'sord_set_fused', 'sord_last_error', 'sord_destroy',
+]
+
+_lib = None
+
+
+class SolverError(RuntimeError):
+    """Error returned by the solver library; ``code`` is the SORD_E* value."""
+
+    def __init__(self, code, msg):
+        self.code = code
+        self.name = ERRORS.get(code, str(code))
+        RuntimeError.__init__(self, 'SORD_%s: %s' % (self.name, msg))
+
+
+def load():
+    """Load libsord_amd.so (once) and declare the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError('HIP solver library not built: %s (python -m coseis_amd.build)' % LIBPATH)
+    lib = ctypes.CDLL(LIBPATH)
+    vp = ctypes.c_void_p
+    lib.sord_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.sord_init.argtypes = [vp]
+    lib.sord_step.argtypes = [vp, ctypes.c_int]
+    lib.sord_run.argtypes = [vp]
+    lib.sord_status.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    lib.sord_field.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), i64p, i64p, i64p]
+    lib.sord_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_int]
+    lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
+    lib.sord_last_error.restype = ctypes.c_char_p
+    lib.sord_last_error.argtypes = []
+    lib.sord_destroy.argtypes = [vp]
+    lib.sord_destroy.restype = None
+    for name in EXPORTS[:-2] + ['sord_set_fused']:
+        getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _lib.sord_last_error().decode(errors='replace')
+        raise SolverError(rc, msg)
+
+
+class Solver(object):
+    """One solver handle: ``sord.in`` text in, reference output files out.
+
+    The current directory at ``init()`` and during stepping receives the
+    output files, as the reference ``sord.x`` writes them into its cwd.
+    """
+
+    def __init__(self, text, realsize=4, device=0, fused=True):
+        lib = load()
+        self._h = ctypes.c_void_p()
+        _check(lib.sord_create(text.encode(), realsize, device, ctypes.byref(self._h)))
+        self.realsize = realsize
+        if not fused:
+            _check(lib.sord_set_fused(self._h, 0))
+        _check(lib.sord_init(self._h))
+
+    def step(self, n=1):
+        _check(_lib.sord_step(self._h, n))
+
+    def run(self):
+        _check(_lib.sord_run(self._h))
+
+    def status(self):
+        it, nt = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.sord_status(self._h, ctypes.byref(it), ctypes.byref(nt)))
+        return it.value, nt.value
+
+    def bench(self, nsteps):
+        tot, ker = ctypes.c_double(), ctypes.c_double()
+        name = ctypes.create_string_buffer(64)
+        _check(_lib.sord_bench(self._h, nsteps, ctypes.byref(tot), ctypes.byref(ker), name, 64))
+        return tot.value, ker.value, name.value.decode()
+
+    def field(self, name):
+        """(device pointer, padded shape, strides, offset) of a device array."""
+        p = ctypes.c_void_p()
+        shp = (ctypes.c_int64 * 3)()
+        st = (ctypes.c_int64 * 3)()
+        off = ctypes.c_int64()
+        _check(_lib.sord_field(self._h, name.encode(), ctypes.byref(p), shp, st, ctypes.byref(off)))
+        return p.value, tuple(shp), tuple(st), off.value
+
+    def close(self):
+        if self._h:
+            _lib.sord_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,87 @@
+/*
+ * C ABI of the MI355X SORD solver library (coseis_amd/libsord_amd.so).
+ *
+ * The reference crosses its drop-in boundary as a process boundary: the
+ * Python stager writes ``sord.in`` and runs ``./sord.x`` as a blocking child
+ * process (cst/sord.py:538-561, cst/job.py:192-194); the Fortran program reads
+ * the text (sord/input_output.f90:21-98), runs the init sequence and the time
+ * loop (sord/sord.f90:24-47) and writes its output files into the current
+ * directory.  These entry points replace that process with a library handle:
+ * same input text, same output files, explicit error codes instead of
+ * ``stop`` (which exits with status 0 under most compilers).
+ *
+ * All functions return 0 on success or one of the SORD_E* codes; the message
+ * of the last error on the calling thread is returned by sord_last_error().
+ * A handle is not re-entrant; distinct handles may run concurrently (each owns
+ * one HIP stream on its device).
+ */
+#ifndef SORD_AMD_H
+#define SORD_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SORD_OK 0
+#define SORD_EINPUT 1       /* malformed sord.in or parameter (sord/input_output.f90:51-92, sord/setup.f90:190) */
+#define SORD_ECFL 2         /* Courant condition not satisfied (sord/material_model.f90:168) */
+#define SORD_EMODULI 3      /* negative moduli (sord/material_model.f90:167) */
+#define SORD_EVOLUME 4      /* negative cell volume (sord/grid_generation.f90:192-198) */
+#define SORD_ENAN 5         /* NaN/Inf in the solution (sord/statistics.f90:52-60) */
+#define SORD_ENANIN 6       /* NaN/Inf in an input file (sord/input_output.f90:333-337) */
+#define SORD_EBADOP 7       /* bad i/o op or time function (sord/input_output.f90:252-254, sord/utilities.f90:279-281) */
+#define SORD_EFILE 8        /* file cannot be read or written (sord/fortran_io.f90:51-92) */
+#define SORD_EHIP 9         /* HIP runtime error */
+#define SORD_EUNSUPPORTED 10 /* input this build does not implement (named in the message) */
+#define SORD_ESTATE 11      /* call out of order */
+
+/* Parse a sord.in text (Appendix format of cst/sord.py:538-544) and bind the
+ * handle to HIP device `device`.  realsize: 4 (float) or 8 (double), the
+ * reference's REALSIZE (sord/Makefile.in:7).  Replaces read_parameters
+ * (sord/input_output.f90:21-98) and setup_dimensions (sord/setup.f90:6-119). */
+int sord_create(const char *sord_in_text, int realsize, int device, void **handle);
+
+/* Initialisation at it = 0: grid, material, PML, rupture, static field i/o,
+ * CFL and moduli checks, device upload.  Replaces sord/sord.f90:30-36. */
+int sord_init(void *handle);
+
+/* Advance `nsteps` time steps (or to the last step) with the per-step field
+ * i/o and statistics.  One iteration of sord/sord.f90:39-47 per step. */
+int sord_step(void *handle, int nsteps);
+
+/* Run all remaining steps and flush every output file (sord/sord.f90:39-49). */
+int sord_run(void *handle);
+
+/* Current step (0 after init) and total steps. */
+int sord_status(void *handle, int *it, int *nt);
+
+/* Borrow a device view of a solver array (the handle keeps ownership).
+ * name: "vx" "vy" "vz" "ux" "uy" "uz" (nodes), "wxx".."wxy" (cell stress of
+ * the last step), "rho"... is not kept on the device.  The element (j,k,l),
+ * 1-based local index over the padded mesh, is at
+ *   ptr[offset + (j-1) + strides[1]*(k-1) + strides[2]*(l-1)].
+ * shape[] receives the padded extents. */
+int sord_field(void *handle, const char *name, void **dev_ptr, int64_t shape[3], int64_t strides[3],
+               int64_t *offset);
+
+/* Time `nsteps` steps with HIP events on the library's stream.  Returns the
+ * wall time of all steps (ms) and the mean duration of one launch of the
+ * dominant solver kernel (ms) with its name.  No field i/o may be due in the
+ * timed window (caller's responsibility: use a run without per-step outputs). */
+int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, char *kernel_name, int name_len);
+
+/* Select the execution path: 1 fused interior kernel where it applies
+ * (default), 0 stage kernels everywhere. */
+int sord_set_fused(void *handle, int on);
+
+/* Message of the last error on this thread ("" if none). */
+const char *sord_last_error(void);
+
+/* Free the handle and its device memory; flushes nothing. */
+void sord_destroy(void *handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

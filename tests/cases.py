@@ -1,0 +1,140 @@
+"""
+Small SORD cases shared by the golden-fixture generator and the tests.
+
+Each case is a function returning (prm, files): the cst.sord parameter
+dictionary and a dict of input files {name: numpy array} to write before the
+run.  Sizes are chosen so that the reference and the oracle finish in about a
+second.  Fault parameters are read from files ('=<'): the reference's
+non-disk ops on fault fields write outside the fault arrays (defect D14), so
+disk input is the form both codes execute the same way.
+"""
+import numpy as np
+
+
+def _src(fields, x, val, tau=0.1, fn='brune'):
+    return {f: [([x, x, x, []], '.', val, fn, tau)] for f in fields}
+
+
+def explosion_pml():
+    """Isotropic moment source in a homogeneous full space, PML on all faces."""
+    n = 21
+    prm = {
+        'shape': [n, n, n, 40], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 4,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0],
+        'itstats': 5, 'itio': 20,
+    }
+    prm.update(_src(['mxx', 'myy', 'mzz'], 10.0, 1e16))
+    prm['vx'] = [([[], [], [], -1], '=>', 'vx.bin'), ([13.0, 10.0, 10.0, []], '.>', 'p1-vx.bin')]
+    prm['vy'] = [([[], [], [], -1], '=>', 'vy.bin'), ([10, 12, 9, []], '=>', 'p2-vy.bin')]
+    prm['vz'] = [([[], [], [], -1], '=>', 'vz.bin')]
+    prm['ux'] = [([[], [], 10, [0, None, 10]], '=>', 'ux-snap.bin')]
+    prm['um2'] = [([[], [], 12, -1], '=>', 'um2.bin')]
+    return prm, {}
+
+
+def layered_free():
+    """Two layers, free surface on top, PML elsewhere, viscous hourglass, viscosity."""
+    nx, ny, nz = 24, 22, 18
+    prm = {
+        'shape': [nx, ny, nz, 50], 'delta': [50.0, 50.0, 50.0, 0.004],
+        'bc1': ['pml', 'pml', 'free'], 'bc2': ['pml'] * 3, 'npml': 5,
+        'rho': [2700.0, ([[], [], [0, 6]], '=', 2600.0)],
+        'vp': [6000.0, ([[], [], [0, 6]], '=', 4000.0)],
+        'vs': [3464.0, ([[], [], [0, 6]], '=', 2000.0)],
+        'gam': [0.05],
+        'hourglass': [1.0, 2.0],
+        'itstats': 10, 'itio': 25,
+        'mxy': [([11.0, 10.0, 8.0, []], '.', 1e16, 'brune', 0.05)],
+    }
+    prm['vx'] = [([[2, 22, 4], [2, 20, 4], 0, []], '=>', 'rcv-vx.bin')]
+    prm['vy'] = [([[2, 22, 4], [2, 20, 4], 0, []], '=>', 'rcv-vy.bin')]
+    prm['vz'] = [([[2, 22, 4], [2, 20, 4], 0, []], '=>', 'rcv-vz.bin'),
+                 ([[], [], [], -1], '=>', 'vz.bin')]
+    prm['wxy'] = [([[], [], 8, 20], '=>', 'wxy.bin')]
+    return prm, {}
+
+
+def fault_sw():
+    """Spontaneous rupture on a +z fault, slip-weakening, nucleation patch."""
+    nx, ny, nz = 25, 21, 20
+    prm = {
+        'shape': [nx, ny, nz, 60], 'delta': [100.0, 100.0, 100.0, 0.008],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 4,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0],
+        'faultnormal': '+z', 'hypocenter': [12.0, 10.0, 9.0],
+        'vrup': -1.0, 'itstats': 10, 'itio': 30,
+    }
+    shp = (nx, ny)
+    mus = np.full(shp, 1e4, 'f4')
+    mus[5:20, 5:16] = 0.677
+    ts = np.full(shp, 70e6, 'f4')
+    ts[10:15, 8:13] = 81.6e6
+    files = {
+        'in-mus.bin': mus, 'in-mud.bin': np.full(shp, 0.525, 'f4'),
+        'in-dc.bin': np.full(shp, 0.4, 'f4'), 'in-tn.bin': np.full(shp, -120e6, 'f4'),
+        'in-ts.bin': ts,
+    }
+    for f in ['mus', 'mud', 'dc', 'tn', 'ts']:
+        prm[f] = [([[], []], '=<', 'in-%s.bin' % f)]
+    for f in ['trup', 'sux', 'suy', 'svm', 'psv', 'sl', 'tsm', 'tnm']:
+        prm[f] = [([[], [], -1], '=>', f + '.bin')]
+    prm['svx'] = [([12, 10, []], '=>', 'p-svx.bin')]
+    prm['tsx'] = [([14, 12, []], '=>', 'p-tsx.bin')]
+    prm['vx'] = [([[], [], [], -1], '=>', 'vx.bin')]
+    return prm, files
+
+
+def stages():
+    """Per-stage fields after a few steps: potency, moment, stress, force,
+    acceleration (the reference's intermediate i/o points)."""
+    n = 14
+    prm = {
+        'shape': [n, n, n, 4], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['pml'] * 3, 'bc2': ['pml', 'free', 'pml'], 'npml': 3,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.1],
+        'hourglass': [1.0, 1.0], 'itstats': 2,
+        'pxx': [([6, 6, 6, 1], '=', 1e-3), ([[], [], 6, [2, 4]], '=>', 'pxx.bin')],
+        'pyz': [([5, 7, 6, [1, 3]], '+', 2e-4), ([[], [], 6, [2, 4]], '=>', 'pyz.bin')],
+        'mxx': [([[], [], 6, [2, 4]], '=>', 'mxx.bin')],
+        'wxx': [([[], [], 6, [2, 4]], '=>', 'wxx.bin')],
+        'wyz': [([[], [], 6, [2, 4]], '=>', 'wyz.bin')],
+        'fx': [([[], [], 6, [2, 4]], '=>', 'fx.bin')],
+        'fz': [([[], [], 7, [2, 4]], '=>', 'fz.bin')],
+        'ax': [([[], [], 6, [2, 4]], '=>', 'ax.bin')],
+        'am2': [([[], [], 6, [2, 4]], '=>', 'am2.bin')],
+        'wm2': [([[], [], 6, [2, 4]], '=>', 'wm2.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin')],
+    }
+    return prm, {}
+
+
+def bc_mix():
+    """Mirror, anti-mirror, rigid and cell-symmetric boundaries, no PML."""
+    n = 16
+    prm = {
+        'shape': [n, n + 1, n + 2, 30], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['+node', '-cell', 'rigid'], 'bc2': ['-node', '+cell', 'free'], 'npml': 0,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.02],
+        'hourglass': [1.0, 1.0], 'itstats': 10,
+        'fx': [([5.0, 6.0, 7.0, [0, 10]], '.', 1e9, 'ricker1', 0.02)],
+        'vy': [([[], [], [], -1], '=>', 'vy.bin')],
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'uz': [([[], [], [], -1], '=>', 'uz.bin')],
+    }
+    return prm, {}
+
+
+CASES = {
+    'explosion_pml': explosion_pml,
+    'layered_free': layered_free,
+    'fault_sw': fault_sw,
+    'stages': stages,
+    'bc_mix': bc_mix,
+}
+
+
+def write_inputs(files, realsize=4):
+    dt = '<f%d' % realsize
+    for name, a in files.items():
+        np.asarray(a, dtype=dt).T.tofile(name)

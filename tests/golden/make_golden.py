@@ -1,0 +1,63 @@
+"""
+Write the golden fixtures of tests/cases.py from the reference solver.
+
+Runs the amdflang build of the reference SORD (oracle/_ref/r{4,8}/sord.x, made
+by oracle/build_ref.sh from /root/reference/sord) on every case in a scratch
+directory and stores every output file it writes as one array of an .npz
+archive per case and precision: tests/golden/<case>_r4.npz, <case>_r8.npz.
+Profile files (timings) are not stored.
+
+    python tests/golden/make_golden.py
+"""
+import glob
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+from coseis_amd import sord  # noqa: E402
+import cases  # noqa: E402
+
+
+def run_reference(case, realsize, exe):
+    prm, files = cases.CASES[case]()
+    d = tempfile.mkdtemp(prefix='golden-')
+    cwd = os.getcwd()
+    try:
+        os.chdir(d)
+        cases.write_inputs(files, realsize)
+        sord.stage(prm, realsize=realsize)
+        subprocess.run([exe], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        out = {}
+        dt = '<f%d' % realsize
+        for f in sorted(glob.glob('*.bin')):
+            if f.startswith('prof-') or f in files:
+                continue
+            out[f[:-4]] = np.fromfile(f, dt)
+        return out
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(d)
+
+
+def main():
+    for rs in (4, 8):
+        exe = os.path.join(ROOT, 'oracle', '_ref', 'r%d' % rs, 'sord.x')
+        if not os.path.exists(exe):
+            sys.exit('reference not built: run oracle/build_ref.sh')
+        for case in cases.CASES:
+            out = run_reference(case, rs, exe)
+            path = os.path.join(HERE, '%s_r%d.npz' % (case, rs))
+            np.savez_compressed(path, **out)
+            print(path, sorted(out))
+
+
+if __name__ == '__main__':
+    main()
