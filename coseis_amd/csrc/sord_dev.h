@@ -97,6 +97,7 @@ struct Fault {
     T *sl, *psv, *trup, *tarr, *svold;
     T *t1[3], *t3[3], *ts, *tn, *fr, *sa[3], *sam;
     T *sv[3], *su[3], *svm, *sum;
+    T *muf, *efacc, *es, *mo;   // energy terms: friction work (accumulated), strain energy, moment
     int faultopening, nucleate;
     T trelax, vrup, rcrit;
 };

@@ -631,15 +631,16 @@ struct Solver {
                 if (o.op[1] == '<') {
                     int lo[3], hi[3];
                     std::vector<T> vals;
+                    const char opc = o.op[0];  // io_read retires the record on its last step
                     if (!io_read(o, x, f.n, lo, hi, vals)) continue;
                     size_t c = 0;
                     for (int l = lo[2]; l <= hi[2]; l++)
                         for (int k = lo[1]; k <= hi[1]; k++)
                             for (int j = lo[0]; j <= hi[0]; j++) {
                                 T v = vals[c++];
-                                if (o.op[0] == '=') f(j, k, l) = v;
-                                else if (o.op[0] == '+') f(j, k, l) = f(j, k, l) + v;
-                                else if (o.op[0] == '*') f(j, k, l) = f(j, k, l) * v;
+                                if (opc == '=') f(j, k, l) = v;
+                                else if (opc == '+') f(j, k, l) = f(j, k, l) + v;
+                                else if (opc == '*') f(j, k, l) = f(j, k, l) * v;
                                 else fail(SORD_EBADOP, "bad i/o op");
                             }
                     bcio(o, f);
@@ -829,7 +830,10 @@ extern "C" int sord_field(void *h, const char *name, void **ptr, int64_t shape[3
 }
 
 extern "C" int sord_set_fused(void *h, int on) {
-    return dispatch(h, [on](auto &S) { S.use_fused = on != 0; });
+    return dispatch(h, [on](auto &S) {
+        S.use_fused = on != 0;
+        if (S.initialized) S.plan_regions();
+    });
 }
 
 extern "C" int sord_bench(void *h, int nsteps, double *ms_total, double *ms_kernel, char *kname, int name_len) {

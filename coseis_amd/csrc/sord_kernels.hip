@@ -626,6 +626,20 @@ __global__ void k_rupture(Params<T> P, Fault<T> F, int it, T tm, int fused) {
             ts = fmin(ts, fr);
             for (int i = 0; i < 3; i++) t1[i] = t3[i] + tn * nh[i];
         }
+        // friction work, strain energy and moment density (sord/dynamic_rupture.f90:353-368)
+        {
+            T dv[3], du[3];
+            for (int i = 0; i < 3; i++) {
+                dv[i] = P.v[i][op] - P.v[i][om];
+                du[i] = P.u[i][op] - P.u[i][om];
+            }
+            const T ef = (t1[0] * dv[0] + t1[1] * dv[1] + t1[2] * dv[2]) * ar;
+            F.efacc[fq] = F.efacc[fq] + P.dt * ef;
+            F.es[fq] = ((F.t0[0][fq] + t1[0]) * du[0] + (F.t0[1][fq] + t1[1]) * du[1] +
+                        (F.t0[2][fq] + t1[2]) * du[2]) *
+                       ar;
+            F.mo[fq] = F.muf[fq] * ar * sqrt(du[0] * du[0] + du[1] * du[1] + du[2] * du[2]);
+        }
         T fm_[3], fp_[3];
         for (int i = 0; i < 3; i++) {
             const T d = ar * (t1[i] - F.t0[i][fq]);

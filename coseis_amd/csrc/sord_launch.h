@@ -21,3 +21,7 @@ template <class T> void launch_fault_time(const Params<T> &P, const Fault<T> &F,
 template <class T> void launch_fault_svold(const Params<T> &P, const Fault<T> &F, hipStream_t s);
 template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, hipStream_t s);
 }  // namespace sord
+
+namespace sord {
+template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const int lo[3], const int hi[3], const T sp[3], int waves, int nz, void *stat, int stat_now, int stat_next, hipStream_t s);
+}
