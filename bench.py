@@ -1,0 +1,174 @@
+"""
+Benchmark of the SORD solver step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the layered half-space model, 321 x 321 x 161 nodes, dx = 50 m,
+dt = 0.004 s, free surface on top, 10-node PML elsewhere, viscous hourglass,
+strike-slip double couple at 2 km, 16 x 16 surface receivers, FP32
+(coseis_amd.configs.layered_321).  The run is staged with W + K time steps;
+W steps run untimed, then exactly K steps are timed (HIP events on the solver
+stream, bracketed by device synchronisation), with their field i/o and
+statistics as the reference performs them.  value = node-updates per second
+over the K steps (nodes nx*ny*nz per step, halo excluded).
+
+roofline: the fused interior kernel (k_fused) is the dominant kernel; its
+algorithmic bytes per node-update are 68 B in FP32 (read u, v: 24 B, write u,
+v: 24 B, cell lambda, mu, yy: 12 B, node 1/m, gamma: 8 B); one launch advances
+the interior box, so achieved = 68 B x nodes / mean launch duration.  traffic:
+HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, read from the
+committed summary in profiles/ when it exists.
+
+cpu_baseline: the reference solver itself (the amdflang -O3 -fopenmp build of
+the reference Fortran, oracle/_ref/r4/sord.x) on this machine's host cores, on
+a bounded sample of the same workload (the same mesh and sources, a few time
+steps, timed from the per-step profile the solver writes); the C/OpenMP oracle
+stands in when the reference build is absent ('port').
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ALG_BYTES = {4: 68, 8: 136}
+HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(ROOT, 'profiles', 'k_fused_layered321_traffic.txt')
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=1500)
+    p.add_argument('--warmup', type=int, default=20)
+    p.add_argument('--realsize', type=int, default=4)
+    p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--cpu-steps', type=int, default=6)
+    p.add_argument('--stage-only', action='store_true', help='stage kernels only (no fused kernel)')
+    return p.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per k_fused launch from the committed rocprofv3 counter summary."""
+    if not os.path.exists(PROFILE):
+        return None
+    m = re.search(r'^traffic_bytes_per_launch:\s*([0-9.eE+]+)', open(PROFILE).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(prm, files, steps):
+    """Time the reference solver (the amdflang build in oracle/_ref, when it was
+    built) or else the oracle on `steps` steps of the same mesh, loop only."""
+    import numpy as np
+    from coseis_amd import configs, sord
+    ref = os.path.join(ROOT, 'oracle', '_ref', 'r4', 'sord.x')
+    kind = 'reference'
+    exe = ref
+    if not os.path.exists(ref):
+        kind = 'port'
+        exe = os.path.join(ROOT, 'oracle', '_build', 'sord_oracle_r4')
+        if not os.path.exists(exe):
+            subprocess.run(['make', '-s', '-C', os.path.join(ROOT, 'oracle')], check=True)
+    cores = min(16, os.cpu_count() or 1)
+    prm = dict(prm)
+    prm['shape'] = prm['shape'][:3] + [steps]
+    d = tempfile.mkdtemp(prefix='sord-cpu-')
+    cwd = os.getcwd()
+    try:
+        os.chdir(d)
+        configs.write_inputs(files)
+        sord.stage(prm)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        t0 = time.perf_counter()
+        subprocess.run([exe], check=True, capture_output=True, text=True, env=env)
+        wall = time.perf_counter() - t0
+        # per-step loop time in ms (statistics.f90 prof-step.bin; the oracle writes it too)
+        ms = np.fromfile('prof-step.bin', '<i4').astype(float)
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(d)
+    nodes = prm['shape'][0] * prm['shape'][1] * prm['shape'][2]
+    loop = ms.sum() / 1e3
+    return {
+        'value': round(nodes * steps / loop / 1e9, 5),
+        'unit': 'Gnode-updates/s',
+        'cores': cores,
+        'kind': kind,
+        'sample': '%d time steps of the same %dx%dx%d mesh and sources, FP32, OMP_NUM_THREADS=%d; '
+                  'loop %.2f s of %.1f s wall' % (steps, prm['shape'][0], prm['shape'][1], prm['shape'][2],
+                                                  cores, loop, wall),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print('bench.py: this build runs on one GPU; a run on %d GPUs is not provided '
+              '(the multi-GPU domain decomposition is design-only)' % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from coseis_amd import configs, solver, sord
+    nt = a.warmup + a.steps
+    prm, files = configs.layered_321(nt=nt)
+    nx, ny, nz = prm['shape'][:3]
+    d = tempfile.mkdtemp(prefix='sord-bench-')
+    cwd = os.getcwd()
+    try:
+        os.chdir(d)
+        configs.write_inputs(files, a.realsize)
+        text = sord.stage(prm, realsize=a.realsize)[0]
+        torch.cuda.init()
+        s = solver.Solver(text, realsize=a.realsize, device=0, fused=not a.stage_only)
+        if a.warmup:
+            s.step(a.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ms_total, ms_kernel, knodes, kname = s.bench(a.steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        s.close()
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(d)
+    nodes = nx * ny * nz
+    secs = ms_total / 1e3
+    value = nodes * a.steps / secs / 1e9
+    alg = ALG_BYTES[a.realsize] * knodes
+    achieved = alg / (ms_kernel / 1e3) / 1e9 if ms_kernel > 0 else None
+    line = {
+        'metric': 'Gnode-updates/s', 'value': round(value, 3), 'unit': 'Gnode-updates/s',
+        'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
+        'ms_per_step': round(ms_total / a.steps, 4),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32' if a.realsize == 4 else 'f64', 'data': 'synthetic',
+        'config': {'workload': 'layered half-space 321x321x161, dx 50 m, dt 0.004 s, free surface + '
+                               '10-node PML, viscous hourglass, double couple, 16x16 receivers',
+                   'nodes': nodes, 'realsize': a.realsize,
+                   'path': 'stage kernels' if a.stage_only else 'fused interior + stage shell'},
+        'wall_s': round(wall, 4),
+        'roofline': {
+            'bound': 'hbm', 'kernel': kname,
+            'achieved': round(achieved, 1) if achieved else None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+            'frac': round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            'traffic': traffic_from_profile(),
+            'alg_bytes_per_launch': alg, 'nodes_per_launch': knodes,
+            'launch_ms': round(ms_kernel, 5),
+        },
+        'hbm_frac_of_step': round(ALG_BYTES[a.realsize] * nodes * a.steps / secs / 1e9 / HBM_PEAK_GBS, 4),
+        'batched_vs_plain': None,
+        'grad_over_forward': None,
+    }
+    if not a.no_cpu:
+        line['cpu_baseline'] = cpu_baseline(prm, files, a.cpu_steps)
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

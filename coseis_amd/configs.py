@@ -1,0 +1,88 @@
+"""
+Benchmark workloads as cst.sord parameter dictionaries.
+
+Every function returns (prm, files): the parameters and a dict of input files
+{name: array} to write before the run.  Indices are 0-based node / cell
+indices of the cst.sord interface.
+"""
+import numpy as np
+
+
+def explosion_61(nt=200):
+    """Explosion in a homogeneous full space: 61^3 nodes, dx = 100 m,
+    dt = 0.0075 s, 10-node PML on all faces, isotropic moment M0 = 1e16 N m
+    spread over the 8 cells around the centre node, Brune pulse tau = 0.1 s."""
+    prm = {
+        'shape': [61, 61, 61, nt], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 10,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0],
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vy': [([[], [], [], -1], '=>', 'vy.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin')],
+    }
+    for f in 'mxx', 'myy', 'mzz':
+        prm[f] = [([30.0, 30.0, 30.0, []], '.', 1e16, 'brune', 0.1)]
+    for f in 'vx', 'vy', 'vz':
+        prm[f].append(([40.0, 30.0, 30.0, []], '.>', 'p1-%s.bin' % f))
+    return prm, {}
+
+
+def layered_321(nt=1500, receivers=True):
+    """A 1 km layer (rho 2600, vp 4000, vs 2000) over a half-space (rho 2700,
+    vp 6000, vs 3464): 321 x 321 x 161 nodes, dx = 50 m, dt = 0.004 s, free
+    surface on top, 10-node PML on the other faces, viscous hourglass on
+    (hourglass = [1, 2]); strike-slip double couple M0 = 1e18 N m at 2 km
+    depth, Brune tau = 0.1 s; velocity on a 16 x 16 surface receiver grid."""
+    layer = ([[], [], [0, 20]], '=')
+    prm = {
+        'shape': [321, 321, 161, nt], 'delta': [50.0, 50.0, 50.0, 0.004],
+        'bc1': ['pml', 'pml', 'free'], 'bc2': ['pml', 'pml', 'pml'], 'npml': 10,
+        'rho': [2700.0, layer + (2600.0,)],
+        'vp': [6000.0, layer + (4000.0,)],
+        'vs': [3464.0, layer + (2000.0,)],
+        'gam': [0.0],
+        'hourglass': [1.0, 2.0],
+        'mxy': [([160.0, 160.0, 40.0, []], '.', 1e18, 'brune', 0.1)],
+    }
+    if receivers:
+        for f in 'vx', 'vy', 'vz':
+            prm[f] = [([[8, 321, 20], [8, 321, 20], 0, []], '=>', 'rcv-%s.bin' % f)]
+    return prm, {}
+
+
+def rupture_400(nt=1500):
+    """Spontaneous rupture on a vertical planar fault (double nodes, normal
+    +y): 400 x 200 x 200 nodes, dx = 100 m, dt = 0.008 s, homogeneous
+    (2670, 6000, 3464), 10-node PML; 30 x 15 km fault with mu_s = 0.677,
+    mu_d = 0.525, dc = 0.4 m, tn = -120 MPa, ts = 70 MPa, a 3 x 3 km
+    nucleation square at 81.6 MPa, mu_s = 1e4 outside the fault.  Fault
+    parameters are read from files (the reference executes non-disk ops on
+    fault fields outside the fault arrays).  Outputs: slip, slip rate,
+    traction and rupture time at the last step."""
+    nx, ny, nz = 400, 200, 200
+    prm = {
+        'shape': [nx, ny, nz, nt], 'delta': [100.0, 100.0, 100.0, 0.008],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 10,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0],
+        'faultnormal': '+y', 'hypocenter': [199.5, 99.0, 99.5], 'vrup': -1.0,
+    }
+    shp = (nx, nz)
+    mus = np.full(shp, 1e4, 'f4')
+    mus[50:350, 25:175] = 0.677
+    ts = np.full(shp, 70e6, 'f4')
+    ts[185:215, 85:115] = 81.6e6
+    files = {
+        'in-mus.bin': mus, 'in-mud.bin': np.full(shp, 0.525, 'f4'),
+        'in-dc.bin': np.full(shp, 0.4, 'f4'), 'in-tn.bin': np.full(shp, -120e6, 'f4'),
+        'in-ts.bin': ts,
+    }
+    for f in ['mus', 'mud', 'dc', 'tn', 'ts']:
+        prm[f] = [([[], []], '=<', 'in-%s.bin' % f)]
+    for f in ['sux', 'suy', 'suz', 'svm', 'tsm', 'trup']:
+        prm[f] = [([[], [], -1], '=>', f + '.bin')]
+    return prm, files
+
+
+def write_inputs(files, realsize=4):
+    for name, a in files.items():
+        np.asarray(a, dtype='<f%d' % realsize).T.tofile(name)

@@ -58,6 +58,7 @@ struct Params {
     int npml;
     T *dn1, *dn2, *dc1, *dc2;   // damping tables, index p-1
     T *p[6], *gm[6];            // node / cell memory slabs: lower x,y,z then upper x,y,z
+    T *gmn[6];                  // second cell-memory buffer: the fused kernels read gm, write gmn
     int pn[6][3], gn[6][3];     // their shapes (Fortran order, per component)
     // hourglass
     int hg_on, hg_pml;
@@ -65,6 +66,17 @@ struct Params {
     // fault
     int ifn, irup;              // 1-based axis (0 none) and local index of the minus side
     int store_a;                // fused update also stores the acceleration (statistics steps)
+    // statistics maxima of squared norms, as bit patterns for atomic max:
+    // stat_cur[0] |a|^2 and [3] ||w||^2 of this step, stat_nxt[1] |v|^2 and
+    // [2] |u|^2 of the state the update produces (next step's statistics)
+    T *stat_cur, *stat_nxt;
+    int stat_now, stat_next;
+};
+
+// One block of the fused kernels: node columns [jb, je] x [kb, ke] (at most
+// 62 x (W-2)) over planes [lz0, lz1]; lz0 > lz1 marks padding.
+struct Work {
+    int jb, je, kb, ke, lz0, lz1, pad0, pad1;
 };
 
 struct Box {

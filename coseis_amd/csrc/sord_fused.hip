@@ -1,44 +1,50 @@
-// Fused single-pass SORD step for interior nodes (CDNA4, gfx950).
+// Fused single-pass SORD step (CDNA4, gfx950).
 //
-// One launch advances every node of a box whose 8 surrounding cells are
-// elastic (no PML memory, no fault, no boundary condition): strain
-// (sord/stress.f90:23-26 with sord/diff_nc_op.f90), Hooke's law and 1/V
-// (sord/stress.f90:192-238), divergence (sord/acceleration.f90:284-293 with
-// sord/diff_cn_op.f90), hourglass stiffness + viscosity
-// (sord/acceleration.f90:390-421 with sord/hourglass_op.f90), Newton
+// One launch advances a set of tiles by one time step: strain
+// (sord/stress.f90:16-165 with sord/diff_nc_op.f90), Hooke's law and 1/V
+// (sord/stress.f90:192-238), divergence (sord/acceleration.f90:280-388 with
+// sord/diff_cn_op.f90), hourglass stiffness and viscosity
+// (sord/acceleration.f90:390-469 with sord/hourglass_op.f90), plane-local
+// boundary conditions (sord/boundary_cond.f90:44-66), Newton
 // (sord/acceleration.f90:498-509) and the leapfrog of the next step
 // (sord/time_integration.f90:26-41), on a mesh with uniform spacing per axis.
 //
-// Work decomposition ("z-march"): a block is W wavefronts; wave w owns node
-// row k = kb - 1 + w, lane i owns node column j = jb - 1 + i, and the block
-// marches the planes l of a chunk.  Lanes 0 and 63 and waves 0 and W-1 are a
-// one-node halo: they load and compute cells but update no node.  Per plane,
-// every lane computes the one cell whose lower corner is its node: its stress
-// and its four hourglass modes.  The cell's contribution to its 8 corner nodes
-// is a polynomial in the corner signs (s_j, s_k, s_l) = +-1:
+// Two instantiations share the decomposition:
+//   k_fused   interior tiles: every cell elastic, every node interior
+//   k_fusedp  the rest: PML slabs (split-field memory g on cells, p on
+//             nodes, viscous-only hourglass at PML nodes), boundary planes
+//             (zeroed halo cells, boundary conditions) and the two fault
+//             planes (force stored for the rupture kernel, no update)
+//
+// Decomposition ("z-march"): a block is W wavefronts; wave w owns node row
+// k = kb - 1 + w, lane i owns node column j = jb - 1 + i, and the block marches
+// the planes l of its work item.  Lanes 0 and 63 and waves 0 and W-1 are a
+// one-node halo: they load and compute cells but update no node.  Per plane
+// every lane computes the one cell whose lower corner is its node.  A cell's
+// contribution to its 8 corner nodes is a polynomial in the corner signs
+// (s_j, s_k, s_l) = +-1 (+1 for the lower corner):
 //     A_j s_j + A_k s_k + A_l s_l + B_jk s_j s_k + B_kl s_k s_l + B_jl s_j s_l + B_jkl s_j s_k s_l
-// (A: divergence of the stress row, B: minus the hourglass modes), so the
-// gather over the 8 cells of a node factorises: the plane l-1 part is carried
-// in registers, the row k-1 part comes through LDS, the column j-1 part
-// through a lane shuffle.  u, v, gamma, 1/m and lambda, mu, yy are read once
-// and u, v written once per node: 68 bytes per node-update in FP32.
+// with A the divergence weights times the stress row and B minus the
+// hourglass modes, so the gather over the 8 cells of a node factorises: the
+// plane l-1 part is carried in registers, the row k-1 part comes through LDS,
+// the column j-1 part through a lane shuffle.  u, v, gamma, 1/m and
+// lambda, mu, yy are read once and u, v written once per node: 68 bytes per
+// node-update in FP32 (plus the PML memory in the slabs).
 #include "sord_dev.h"
 #include "sord_launch.h"
 
 namespace sord {
 
 template <class T>
-__device__ __forceinline__ void atomic_max_bits(void *slot, T v);
+__device__ __forceinline__ void atomic_max_bits(T *slot, T v);
 template <>
-__device__ __forceinline__ void atomic_max_bits<float>(void *slot, float v) {
-    // v >= 0 or NaN: the ordering of the magnitude bits is the ordering of the
-    // values, and NaN sorts above +inf
+__device__ __forceinline__ void atomic_max_bits<float>(float *slot, float v) {
+    // v >= 0 or NaN: the magnitude bits order like the values, NaN above +inf
     atomicMax((unsigned int *)slot, __float_as_uint(v) & 0x7fffffffu);
 }
 template <>
-__device__ __forceinline__ void atomic_max_bits<double>(void *slot, double v) {
-    unsigned long long b = (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull;
-    atomicMax((unsigned long long *)slot, b);
+__device__ __forceinline__ void atomic_max_bits<double>(double *slot, double v) {
+    atomicMax((unsigned long long *)slot, (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull);
 }
 
 template <class T>
@@ -49,46 +55,96 @@ __device__ __forceinline__ T wave_max(T v) {
     }
     return v;
 }
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) {
+    return (b > a || b != b) ? b : a;
+}
 
 template <class T>
-struct FusedP {
-    T *u[3], *v[3], *un[3], *vn[3];
-    T *gam, *mr, *lam, *mu, *yy;
-    Grid<T> g;
-    int lo[3], hi[3];   // node box (1-based local, inclusive)
-    int nz;             // planes per block
-    int nbx, nby, nbz;  // blocks per axis
-    T gx, gy, gz;       // 1 / (4 dx_a): gradient from the sum of four edge differences
-    T ax, ay, az;       // dx_b dx_c / 4: divergence weights
-    T h1, dth2, dt;
-    T vol, rvol;        // cell volume and its inverse (source ops)
-    void *stat;         // 4 max slots (a, v, u, w squared norms) or null
-    int stat_now, stat_next;
+struct FusedC {
+    const Work *work;
+    T gx, gy, gz;  // 1 / (4 dx_a): gradient from the sum of four edge differences
+    T ax, ay, az;  // dx_b dx_c / 4: divergence weights
+    T h1, dth2;    // hourglass(1), dt * hourglass(2)
+    T vol, rvol;   // cell volume and its inverse
 };
 
+// ---------------------------------------------------------------------------
+// shared pieces
+// ---------------------------------------------------------------------------
+
+// source ops on one cell in the reference's units: volume-integrated strain
+// (potency), moment, stress; G is the true gradient, returns the stress
+template <class T>
+__device__ __forceinline__ void cell_ops(const PointOps<T> &ops, long o, const T G[3][3], T lm, T m, T vol,
+                                         T rvol, T s[6]) {
+    T w1[3], w2[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) w1[i] = G[i][i] * vol;
+    w2[0] = T(0.5) * (G[2][1] * vol) + T(0.5) * (G[1][2] * vol);
+    w2[1] = T(0.5) * (G[0][2] * vol) + T(0.5) * (G[2][0] * vol);
+    w2[2] = T(0.5) * (G[1][0] * vol) + T(0.5) * (G[0][1] * vol);
+    auto apply = [&](int stage) {
+        for (int q = 0; q < ops.n; q++) {
+            if (ops.stage[q] != stage || ops.cell[q] != o) continue;
+            const int cc = ops.comp[q], opq = ops.op[q];
+            const T x = ops.val[q];
+#pragma unroll
+            for (int t = 0; t < 6; t++) {
+                if (t != cc) continue;
+                T &r = t < 3 ? w1[t] : w2[t - 3];
+                r = opq == 0 ? x : opq == 1 ? r + x : r * x;
+            }
+        }
+    };
+    apply(0);
+    const T tr = lm * (w1[0] + w1[1] + w1[2]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        w1[i] = T(2) * m * w1[i] + tr;
+        w2[i] = T(2) * m * w2[i];
+    }
+    apply(1);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        w1[i] = w1[i] * rvol;
+        w2[i] = w2[i] * rvol;
+    }
+    apply(2);
+    s[0] = w1[0], s[1] = w1[1], s[2] = w1[2], s[3] = w2[0], s[4] = w2[1], s[5] = w2[2];
+}
+
+// the four hourglass modes of one component over the cell corners
+// (F*lo: corners (dj,dk) of plane l, F*hi of plane l+1, index dj + 2 dk)
+template <class T>
+__device__ __forceinline__ void hg_modes(T f00, T f10, T f01, T f11, T g00, T g10, T g01, T g11, T h[4]) {
+    const T a00 = f00 - f10, a10 = f01 - f11;  // along j, at (dk, dl) = (0,0), (1,0)
+    const T a01 = g00 - g10, a11 = g01 - g11;  // (0,1), (1,1)
+    const T b00 = f00 + f10, b10 = f01 + f11;
+    const T b01 = g00 + g10, b11 = g01 + g11;
+    h[0] = (b00 - b10) - (b01 - b11);  // s_k s_l
+    h[1] = (a00 + a10) - (a01 + a11);  // s_j s_l
+    const T d0 = a00 - a10, d1 = a01 - a11;
+    h[2] = d0 + d1;  // s_j s_k
+    h[3] = d0 - d1;  // s_j s_k s_l
+}
+
+// ---------------------------------------------------------------------------
+// interior tiles
+// ---------------------------------------------------------------------------
 template <class T, int W>
-__global__ __launch_bounds__(64 * W) void k_fused(FusedP<T> Q, PointOps<T> ops) {
+__global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     __shared__ T sN[6][W][64];
     __shared__ T sC[6][W][64];
     const int lane = threadIdx.x, w = threadIdx.y;
-    // block -> tile: consecutive tiles along k share halo rows, so map tiles
-    // with the same hardware XCD (round-robin of the linear block id over 8
-    // XCDs) to neighbouring k positions
-    const int nblk = Q.nbx * Q.nby * Q.nbz;
-    const int bid = blockIdx.x;
-    int t = bid;
-    if (nblk % 8 == 0) t = (bid % 8) * (nblk / 8) + bid / 8;
-    const int by = t % Q.nby, bx = (t / Q.nby) % Q.nbx, bz = t / (Q.nby * Q.nbx);
-    const int jb = Q.lo[0] + bx * 62, kb = Q.lo[1] + by * (W - 2);
-    const int lz0 = Q.lo[2] + bz * Q.nz;
-    if (lz0 > Q.hi[2]) return;  // uniform over the block
-    const int lz1 = min(lz0 + Q.nz - 1, Q.hi[2]);
-    const int j = jb - 1 + lane, k = kb - 1 + w;
-    const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= Q.hi[0] && k <= Q.hi[1];
-    const bool cellv = lane <= 62 && w <= W - 2 && j <= Q.hi[0] && k <= Q.hi[1];
-    const Grid<T> &g = Q.g;
+    const Work wk = C.work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;  // uniform over the block
+    const int j = wk.jb - 1 + lane, k = wk.kb - 1 + w;
+    const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= wk.je && k <= wk.ke;
+    const bool cellv = lane <= 62 && w <= W - 2 && j <= wk.je && k <= wk.ke;
+    const Grid<T> &g = P.g;
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
-    const bool stat_now = Q.stat && Q.stat_now, stat_next = Q.stat && Q.stat_next;
+    const bool stat_now = P.stat_now, stat_next = P.stat_next;
     T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
 
     // node data: W = u + gamma v (strain), F = h1 u + dt h2 v (hourglass)
@@ -97,13 +153,13 @@ __global__ __launch_bounds__(64 * W) void k_fused(FusedP<T> Q, PointOps<T> ops) 
 
     auto load_node = [&](int l, T own6[6], T uu[3], T vv[3]) {
         const long o = g.at(jc, kc, l);
-        const T ga = Q.gam[o];
+        const T ga = P.gam[o];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            uu[i] = Q.u[i][o];
-            vv[i] = Q.v[i][o];
+            uu[i] = P.u[i][o];
+            vv[i] = P.v[i][o];
             own6[i] = uu[i] + ga * vv[i];
-            own6[3 + i] = Q.h1 * uu[i] + Q.dth2 * vv[i];
+            own6[3 + i] = C.h1 * uu[i] + C.dth2 * vv[i];
         }
     };
     auto exchange = [&](const T own6[6], T N[4][6]) {
@@ -120,116 +176,72 @@ __global__ __launch_bounds__(64 * W) void k_fused(FusedP<T> Q, PointOps<T> ops) 
             N[3][q] = __shfl_down(ab, 1);
         }
     };
-    // stress and hourglass of cell (j, k, l) from corner planes Nlo (l) and Nhi (l+1);
-    // E: polynomial evaluated at s_l = +1, C: at s_l = -1
-    auto cell = [&](int l, T E[3][4], T C[3][4]) {
+    // stress and hourglass of cell (j, k, l); E: polynomial at s_l = +1, Cc: at s_l = -1
+    auto cell = [&](int l, T E[3][4], T Cc[3][4]) {
         const long oc = g.at(jc, kc, l);
-        const T lm = Q.lam[oc], m = Q.mu[oc], y = Q.yy[oc];
+        const T lm = P.lam[oc], m = P.mu[oc], y = P.yy[oc];
         T G[3][3];  // G[a][c]: gradient along a of component c
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            G[0][c] = Q.gx * (((Nlo[1][c] - Nlo[0][c]) + (Nlo[3][c] - Nlo[2][c])) +
+            G[0][c] = C.gx * (((Nlo[1][c] - Nlo[0][c]) + (Nlo[3][c] - Nlo[2][c])) +
                               ((Nhi[1][c] - Nhi[0][c]) + (Nhi[3][c] - Nhi[2][c])));
-            G[1][c] = Q.gy * (((Nlo[2][c] - Nlo[0][c]) + (Nlo[3][c] - Nlo[1][c])) +
+            G[1][c] = C.gy * (((Nlo[2][c] - Nlo[0][c]) + (Nlo[3][c] - Nlo[1][c])) +
                               ((Nhi[2][c] - Nhi[0][c]) + (Nhi[3][c] - Nhi[1][c])));
-            G[2][c] = Q.gz * (((Nhi[0][c] - Nlo[0][c]) + (Nhi[1][c] - Nlo[1][c])) +
+            G[2][c] = C.gz * (((Nhi[0][c] - Nlo[0][c]) + (Nhi[1][c] - Nlo[1][c])) +
                               ((Nhi[2][c] - Nlo[2][c]) + (Nhi[3][c] - Nlo[3][c])));
         }
-        T sxx, syy, szz, syz, szx, sxy;
-        const long o = g.at(j, k, l);
+        T s[6];
         const bool inbox = ops.n > 0 && cellv && j >= ops.lo[0] && j <= ops.hi[0] && k >= ops.lo[1] &&
                            k <= ops.hi[1] && l >= ops.lo[2] && l <= ops.hi[2];
         if (!inbox) {
             const T tr = G[0][0] + G[1][1] + G[2][2];
             const T m2 = m + m;
-            sxx = m2 * G[0][0] + lm * tr;
-            syy = m2 * G[1][1] + lm * tr;
-            szz = m2 * G[2][2] + lm * tr;
-            syz = m * (G[2][1] + G[1][2]);
-            szx = m * (G[0][2] + G[2][0]);
-            sxy = m * (G[1][0] + G[0][1]);
+            s[0] = m2 * G[0][0] + lm * tr;
+            s[1] = m2 * G[1][1] + lm * tr;
+            s[2] = m2 * G[2][2] + lm * tr;
+            s[3] = m * (G[2][1] + G[1][2]);
+            s[4] = m * (G[0][2] + G[2][0]);
+            s[5] = m * (G[1][0] + G[0][1]);
         } else {
-            // reference units: volume-integrated strain, potency / moment / stress ops
-            T w1[3], w2[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) w1[i] = G[i][i] * Q.vol;
-            w2[0] = T(0.5) * (G[2][1] * Q.vol) + T(0.5) * (G[1][2] * Q.vol);
-            w2[1] = T(0.5) * (G[0][2] * Q.vol) + T(0.5) * (G[2][0] * Q.vol);
-            w2[2] = T(0.5) * (G[1][0] * Q.vol) + T(0.5) * (G[0][1] * Q.vol);
-            auto apply = [&](int stage) {
-                for (int q = 0; q < ops.n; q++) {
-                    if (ops.stage[q] != stage || ops.cell[q] != o) continue;
-                    const int cc = ops.comp[q], opq = ops.op[q];
-                    const T x = ops.val[q];
-#pragma unroll
-                    for (int t2 = 0; t2 < 6; t2++) {
-                        if (t2 != cc) continue;
-                        T &r = t2 < 3 ? w1[t2] : w2[t2 - 3];
-                        r = opq == 0 ? x : opq == 1 ? r + x : r * x;
-                    }
-                }
-            };
-            apply(0);
-            const T tr = lm * (w1[0] + w1[1] + w1[2]);
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                w1[i] = T(2) * m * w1[i] + tr;
-                w2[i] = T(2) * m * w2[i];
-            }
-            apply(1);
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                w1[i] = w1[i] * Q.rvol;
-                w2[i] = w2[i] * Q.rvol;
-            }
-            apply(2);
-            sxx = w1[0], syy = w1[1], szz = w1[2], syz = w2[0], szx = w2[1], sxy = w2[2];
+            cell_ops<T>(ops, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
         }
-        if (stat_now && cellv) {
-            const T n2 = (sxx * sxx + syy * syy + szz * szz) + T(2) * (syz * syz + szx * szx + sxy * sxy);
-            st_w = (n2 > st_w || n2 != n2) ? n2 : st_w;
-        }
-        const T A[3][3] = {{Q.ax * sxx, Q.ay * sxy, Q.az * szx},
-                           {Q.ax * sxy, Q.ay * syy, Q.az * syz},
-                           {Q.ax * szx, Q.ay * syz, Q.az * szz}};
+        if (stat_now && cellv)
+            st_w = tmax(st_w, (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + T(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+        const T A[3][3] = {{C.ax * s[0], C.ay * s[5], C.az * s[4]},
+                           {C.ax * s[5], C.ay * s[1], C.az * s[3]},
+                           {C.ax * s[4], C.ay * s[3], C.az * s[2]}};
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            // hourglass modes over the corners (d_j, d_k, d_l), sign +1 for the lower corner
-            const T a00 = Nlo[0][3 + c] - Nlo[1][3 + c], a10 = Nlo[2][3 + c] - Nlo[3][3 + c];
-            const T a01 = Nhi[0][3 + c] - Nhi[1][3 + c], a11 = Nhi[2][3 + c] - Nhi[3][3 + c];
-            const T b00 = Nlo[0][3 + c] + Nlo[1][3 + c], b10 = Nlo[2][3 + c] + Nlo[3][3 + c];
-            const T b01 = Nhi[0][3 + c] + Nhi[1][3 + c], b11 = Nhi[2][3 + c] + Nhi[3][3 + c];
-            const T h1 = y * ((b00 - b10) - (b01 - b11));  // s_k s_l
-            const T h2 = y * ((a00 + a10) - (a01 + a11));  // s_j s_l
-            const T d0 = a00 - a10, d1 = a01 - a11;
-            const T h3 = y * (d0 + d1);                    // s_j s_k
-            const T h4 = y * (d0 - d1);                    // s_j s_k s_l
-            const T Aj = A[c][0], Ak = A[c][1], Al = A[c][2];
-            E[c][0] = Al;
-            E[c][1] = Aj - h2;
-            E[c][2] = Ak - h1;
-            E[c][3] = -h3 - h4;
-            C[c][0] = -Al;
-            C[c][1] = Aj + h2;
-            C[c][2] = Ak + h1;
-            C[c][3] = -h3 + h4;
+            T h[4];
+            hg_modes<T>(Nlo[0][3 + c], Nlo[1][3 + c], Nlo[2][3 + c], Nlo[3][3 + c], Nhi[0][3 + c], Nhi[1][3 + c],
+                        Nhi[2][3 + c], Nhi[3][3 + c], h);
+#pragma unroll
+            for (int q = 0; q < 4; q++) h[q] = y * h[q];
+            E[c][0] = A[c][2];
+            E[c][1] = A[c][0] - h[1];
+            E[c][2] = A[c][1] - h[0];
+            E[c][3] = -h[2] - h[3];
+            Cc[c][0] = -A[c][2];
+            Cc[c][1] = A[c][0] + h[1];
+            Cc[c][2] = A[c][1] + h[0];
+            Cc[c][3] = -h[2] + h[3];
         }
     };
 
     T own6[6], carry[3][4], E[3][4], Cn[3][4];
-    load_node(lz0 - 1, own6, uo, vo);
+    load_node(wk.lz0 - 1, own6, uo, vo);
     exchange(own6, Nlo);
-    load_node(lz0, own6, uo, vo);
+    load_node(wk.lz0, own6, uo, vo);
+    __syncthreads();  // every wave has read the first plane from LDS
     exchange(own6, Nhi);
-    cell(lz0 - 1, E, carry);
+    cell(wk.lz0 - 1, E, carry);
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
         for (int c = 0; c < 6; c++) Nlo[q][c] = Nhi[q][c];
 
-    for (int l = lz0; l <= lz1; l++) {
-        const long on = g.at(jc, kc, l);
-        const T mrn = Q.mr[on];
+    for (int l = wk.lz0; l <= wk.lz1; l++) {
+        const T mrn = P.mr[g.at(jc, kc, l)];
         load_node(l + 1, own6, un1, vn1);
         exchange(own6, Nhi);
         cell(l, E, Cn);
@@ -252,8 +264,7 @@ __global__ __launch_bounds__(64 * W) void k_fused(FusedP<T> Q, PointOps<T> ops) 
         for (int c = 0; c < 3; c++) {
             const T r0 = R[c][0] + sC[2 * c][wb][lane];
             const T rj = R[c][1] + sC[2 * c + 1][wb][lane];
-            const T send = r0 - rj;
-            f[c] = (r0 + rj) + __shfl_up(send, 1);
+            f[c] = (r0 + rj) + __shfl_up(r0 - rj, 1);
         }
         if (own) {
             const long o = g.at(j, k, l);
@@ -261,19 +272,16 @@ __global__ __launch_bounds__(64 * W) void k_fused(FusedP<T> Q, PointOps<T> ops) 
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const T a = f[c] * mrn;
-                const T vv = vo[c] + Q.dt * a;
-                const T uu = uo[c] + Q.dt * vv;
-                Q.vn[c][o] = vv;
-                Q.un[c][o] = uu;
+                const T vv = vo[c] + P.dt * a;
+                const T uu = uo[c] + P.dt * vv;
+                P.vn[c][o] = vv;
+                P.un[c][o] = uu;
                 a2 = a2 + a * a;
                 v2 = v2 + vv * vv;
                 u2 = u2 + uu * uu;
             }
-            if (stat_now) st_a = (a2 > st_a || a2 != a2) ? a2 : st_a;
-            if (stat_next) {
-                st_v = (v2 > st_v || v2 != v2) ? v2 : st_v;
-                st_u = (u2 > st_u || u2 != u2) ? u2 : st_u;
-            }
+            if (stat_now) st_a = tmax(st_a, a2);
+            if (stat_next) st_v = tmax(st_v, v2), st_u = tmax(st_u, u2);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) uo[c] = un1[c], vo[c] = vn1[c];
@@ -282,68 +290,392 @@ __global__ __launch_bounds__(64 * W) void k_fused(FusedP<T> Q, PointOps<T> ops) 
 #pragma unroll
             for (int c = 0; c < 6; c++) Nlo[q][c] = Nhi[q][c];
     }
-    if (Q.stat) {
-        T *slot = (T *)Q.stat;
-        if (stat_now) {
-            st_a = wave_max(st_a);
-            st_w = wave_max(st_w);
-            if (lane == 0) {
-                atomic_max_bits<T>(slot + 0, st_a);
-                atomic_max_bits<T>(slot + 3, st_w);
+    if (stat_now) {
+        st_a = wave_max(st_a);
+        st_w = wave_max(st_w);
+        if (lane == 0) atomic_max_bits<T>(P.stat_cur + 0, st_a), atomic_max_bits<T>(P.stat_cur + 3, st_w);
+    }
+    if (stat_next) {
+        st_v = wave_max(st_v);
+        st_u = wave_max(st_u);
+        if (lane == 0) atomic_max_bits<T>(P.stat_nxt + 1, st_v), atomic_max_bits<T>(P.stat_nxt + 2, st_u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PML, boundary and fault tiles
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ long slab_index(const int *n, int i, int j, int k, int comp) {
+    return (long)(i - 1) + (long)n[0] * ((long)(j - 1) + (long)n[1] * (long)(k - 1)) +
+           (long)comp * n[0] * n[1] * n[2];
+}
+
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+    __shared__ T sN[7][W][64];
+    __shared__ T sC[21][W][64];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const Work wk = C.work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;  // uniform over the block
+    const int j = wk.jb - 1 + lane, k = wk.kb - 1 + w;
+    const bool ownjk = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= wk.je && k <= wk.ke;
+    const bool cellv = lane <= 62 && w <= W - 2 && j <= wk.je && k <= wk.ke;
+    const Grid<T> &g = P.g;
+    const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
+    const bool stat_now = P.stat_now, stat_next = P.stat_next;
+    T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
+    const int fa = P.ifn - 1;  // fault normal axis (-1: none)
+
+    // node data per corner: u (3), v (3), gamma
+    T Ulo[4][3], Vlo[4][3], Glo[4], Uhi[4][3], Vhi[4][3], Ghi[4];
+    T uo[3], vo[3], un1[3], vn1[3];
+
+    auto load_node = [&](int l, T own7[7], T uu[3], T vv[3]) {
+        const long o = g.at(jc, kc, l);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            uu[i] = P.u[i][o];
+            vv[i] = P.v[i][o];
+            own7[i] = uu[i];
+            own7[3 + i] = vv[i];
+        }
+        own7[6] = P.gam[o];
+    };
+    auto exchange = [&](const T own7[7], T U[4][3], T V[4][3], T G[4]) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) sN[q][w][lane] = own7[q];
+        __syncthreads();
+        const int wa = w + 1 < W ? w + 1 : w;
+        T x[4][7];
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            const T ab = sN[q][wa][lane];
+            x[0][q] = own7[q];
+            x[2][q] = ab;
+            x[1][q] = __shfl_down(own7[q], 1);
+            x[3][q] = __shfl_down(ab, 1);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) U[c][i] = x[c][i], V[c][i] = x[c][3 + i];
+            G[c] = x[c][6];
+        }
+    };
+    // cell (j, k, l): polynomial coefficients per component, at s_l = +1 (E) and
+    // -1 (Cc): [0..2] divergence along x, y, z (z-summed values with the sign of
+    // s_l applied to the z term), [3..5] full hourglass (s_j, s_k, s_j s_k), [6..8]
+    // viscous hourglass
+    // wr: this block owns the cell and writes its PML memory.  A cell on a tile
+    // edge is also computed by the neighbouring tile (as its halo), so every
+    // block reads the memory of the previous step from gm and only the owner
+    // writes the new value, to gmn; the host swaps the two after the launch.
+    auto cell = [&](int l, T E[3][9], T Cc[3][9], bool wr) {
+        const int c3[3] = {j, k, l};
+        bool real = true;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (c3[a] < P.i1cell[a] || c3[a] > P.i2cell[a]) real = false;
+        if (fa >= 0 && c3[fa] == P.irup) real = false;  // zero-width fault cell: 1/V = 0, yy = 0
+        const long oc = g.at(jc, kc, l);
+        const T lm = P.lam[oc], m = P.mu[oc], y = P.yy[oc];
+        bool inLo[3], inHi[3], inS[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            inLo[a] = c3[a] <= P.i1pml[a];
+            inHi[a] = c3[a] >= P.i2pml[a] - 1;
+            inS[a] = inLo[a] || inHi[a];
+        }
+        const T gsp[3] = {C.gx, C.gy, C.gz};
+        T G[3][3];  // G[id][ic]
+#pragma unroll
+        for (int id = 0; id < 3; id++) {
+            bool other = false;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if (a != id && inS[a]) other = true;
+            const bool useV = inS[id], useU = !inS[id] && other;
+#pragma unroll
+            for (int ic = 0; ic < 3; ic++) {
+                // corner values of the source field (v in the damped direction,
+                // u in other PML directions, u + gamma v elsewhere)
+                T f[8];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    f[q] = useV ? Vlo[q][ic] : useU ? Ulo[q][ic] : Ulo[q][ic] + Glo[q] * Vlo[q][ic];
+                    f[4 + q] = useV ? Vhi[q][ic] : useU ? Uhi[q][ic] : Uhi[q][ic] + Ghi[q] * Vhi[q][ic];
+                }
+                T d;
+                if (id == 0) d = ((f[1] - f[0]) + (f[3] - f[2])) + ((f[5] - f[4]) + (f[7] - f[6]));
+                else if (id == 1) d = ((f[2] - f[0]) + (f[3] - f[1])) + ((f[6] - f[4]) + (f[7] - f[5]));
+                else d = ((f[4] - f[0]) + (f[5] - f[1])) + ((f[6] - f[2]) + (f[7] - f[3]));
+                T gr = gsp[id] * d;
+                if (useV && real) {
+                    // split-field memory in the reference's units (volume-integrated strain)
+                    int si, pidx, sl;
+                    if (inLo[id]) {
+                        si = c3[id] - P.i1cell[id] + 1;
+                        pidx = c3[id] + g.nnoff[id];
+                        sl = id;
+                    } else {
+                        si = P.i2cell[id] - c3[id] + 1;
+                        pidx = g.nn[id] - c3[id] - g.nnoff[id];
+                        sl = id + 3;
+                    }
+                    int sj = j, sk = k, sll = l;
+                    if (id == 0) sj = si;
+                    if (id == 1) sk = si;
+                    if (id == 2) sll = si;
+                    const long go = slab_index<T>(P.gn[sl], sj, sk, sll, ic);
+                    const T dv = P.dc2[pidx - 1] * (gr * C.vol) + P.dc1[pidx - 1] * P.gm[sl][go];
+                    if (wr) P.gmn[sl][go] = dv;
+                    gr = dv * C.rvol;
+                }
+                G[id][ic] = gr;
             }
         }
-        if (stat_next) {
-            st_v = wave_max(st_v);
-            st_u = wave_max(st_u);
-            if (lane == 0) {
-                atomic_max_bits<T>(slot + 1, st_v);
-                atomic_max_bits<T>(slot + 2, st_u);
+        T s[6];
+        const bool inbox = ops.n > 0 && cellv && j >= ops.lo[0] && j <= ops.hi[0] && k >= ops.lo[1] &&
+                           k <= ops.hi[1] && l >= ops.lo[2] && l <= ops.hi[2];
+        if (!inbox) {
+            const T tr = G[0][0] + G[1][1] + G[2][2];
+            const T m2 = m + m;
+            s[0] = m2 * G[0][0] + lm * tr;
+            s[1] = m2 * G[1][1] + lm * tr;
+            s[2] = m2 * G[2][2] + lm * tr;
+            s[3] = m * (G[2][1] + G[1][2]);
+            s[4] = m * (G[0][2] + G[2][0]);
+            s[5] = m * (G[1][0] + G[0][1]);
+        } else {
+            cell_ops<T>(ops, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
+        }
+        if (!real)
+#pragma unroll
+            for (int q = 0; q < 6; q++) s[q] = T(0);
+        if (stat_now && cellv)
+            st_w = tmax(st_w, (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + T(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+        const T A[3][3] = {{C.ax * s[0], C.ay * s[5], C.az * s[4]},
+                           {C.ax * s[5], C.ay * s[1], C.az * s[3]},
+                           {C.ax * s[4], C.ay * s[3], C.az * s[2]}};
+        const T yf = real ? y : T(0), yv = real ? C.dth2 * y : T(0);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T h[4], hv[4];
+            hg_modes<T>(C.h1 * Ulo[0][c] + C.dth2 * Vlo[0][c], C.h1 * Ulo[1][c] + C.dth2 * Vlo[1][c],
+                        C.h1 * Ulo[2][c] + C.dth2 * Vlo[2][c], C.h1 * Ulo[3][c] + C.dth2 * Vlo[3][c],
+                        C.h1 * Uhi[0][c] + C.dth2 * Vhi[0][c], C.h1 * Uhi[1][c] + C.dth2 * Vhi[1][c],
+                        C.h1 * Uhi[2][c] + C.dth2 * Vhi[2][c], C.h1 * Uhi[3][c] + C.dth2 * Vhi[3][c], h);
+            hg_modes<T>(Vlo[0][c], Vlo[1][c], Vlo[2][c], Vlo[3][c], Vhi[0][c], Vhi[1][c], Vhi[2][c], Vhi[3][c], hv);
+#pragma unroll
+            for (int q = 0; q < 4; q++) h[q] = yf * h[q], hv[q] = yv * hv[q];
+            E[c][0] = A[c][0];
+            E[c][1] = A[c][1];
+            E[c][2] = A[c][2];
+            Cc[c][0] = A[c][0];
+            Cc[c][1] = A[c][1];
+            Cc[c][2] = -A[c][2];
+            E[c][3] = -h[1], E[c][4] = -h[0], E[c][5] = -h[2] - h[3];
+            Cc[c][3] = h[1], Cc[c][4] = h[0], Cc[c][5] = -h[2] + h[3];
+            E[c][6] = -hv[1], E[c][7] = -hv[0], E[c][8] = -hv[2] - hv[3];
+            Cc[c][6] = hv[1], Cc[c][7] = hv[0], Cc[c][8] = -hv[2] + hv[3];
+        }
+    };
+
+    T own7[7], carry[3][9], E[3][9], Cn[3][9];
+    load_node(wk.lz0 - 1, own7, uo, vo);
+    exchange(own7, Ulo, Vlo, Glo);
+    load_node(wk.lz0, own7, uo, vo);
+    __syncthreads();  // every wave has read the first plane from LDS
+    exchange(own7, Uhi, Vhi, Ghi);
+    cell(wk.lz0 - 1, E, carry, false);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) Ulo[q][c] = Uhi[q][c], Vlo[q][c] = Vhi[q][c];
+        Glo[q] = Ghi[q];
+    }
+
+    for (int l = wk.lz0; l <= wk.lz1; l++) {
+        const T mrn = P.mr[g.at(jc, kc, l)];
+        load_node(l + 1, own7, un1, vn1);
+        exchange(own7, Uhi, Vhi, Ghi);
+        cell(l, E, Cn, ownjk);
+        // plane combination, then the k stage: own part (s_k = +1) and the part
+        // sent to the row above (s_k = -1): divergence x, y, z, full (c0, cj), viscous (c0, cj)
+        T R[3][7];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T q[9];
+#pragma unroll
+            for (int t = 0; t < 9; t++) q[t] = E[c][t] + carry[c][t];
+            R[c][0] = q[0];
+            R[c][1] = q[1];
+            R[c][2] = q[2];
+            R[c][3] = q[4];          // full hourglass: constant = s_k coefficient
+            R[c][4] = q[3] + q[5];   // s_j coefficient
+            R[c][5] = q[7];
+            R[c][6] = q[6] + q[8];
+            sC[7 * c + 0][w][lane] = q[0];
+            sC[7 * c + 1][w][lane] = -q[1];
+            sC[7 * c + 2][w][lane] = q[2];
+            sC[7 * c + 3][w][lane] = -q[4];
+            sC[7 * c + 4][w][lane] = q[3] - q[5];
+            sC[7 * c + 5][w][lane] = -q[7];
+            sC[7 * c + 6][w][lane] = q[6] - q[8];
+#pragma unroll
+            for (int t = 0; t < 9; t++) carry[c][t] = Cn[c][t];
+        }
+        __syncthreads();
+        const int wb = w > 0 ? w - 1 : 0;
+        // j stage: divergences per direction and the two hourglass variants
+        T D[3][3], Hf[3], Hv[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T r[7];
+#pragma unroll
+            for (int t = 0; t < 7; t++) r[t] = R[c][t] + sC[7 * c + t][wb][lane];
+            D[c][0] = r[0] + __shfl_up(-r[0], 1);
+            D[c][1] = r[1] + __shfl_up(r[1], 1);
+            D[c][2] = r[2] + __shfl_up(r[2], 1);
+            Hf[c] = (r[3] + r[4]) + __shfl_up(r[3] - r[4], 1);
+            Hv[c] = (r[5] + r[6]) + __shfl_up(r[5] - r[6], 1);
+        }
+        const int n3[3] = {j, k, l};
+        bool own = ownjk;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (n3[a] < P.i1node[a] || n3[a] > P.i2node[a]) own = false;
+        if (own) {
+            const long o = g.at(j, k, l);
+            T f[3];
+#pragma unroll
+            for (int ic = 0; ic < 3; ic++) {
+                T acc = T(0);
+#pragma unroll
+                for (int iid = 0; iid < 3; iid++) {
+                    const int id = (ic + iid) % 3;
+                    T sv = D[ic][id];
+                    // PML momentum memory (sord/acceleration.f90:295-362)
+                    if (n3[id] <= P.i1pml[id]) {
+                        const int si = n3[id] - P.i1node[id] + 1, pidx = n3[id] + g.nnoff[id];
+                        int sj = j, sk = k, sl = l;
+                        if (id == 0) sj = si;
+                        if (id == 1) sk = si;
+                        if (id == 2) sl = si;
+                        T *pp = P.p[id];
+                        const long po = slab_index<T>(P.pn[id], sj, sk, sl, ic);
+                        sv = P.dn2[pidx - 1] * sv + P.dn1[pidx - 1] * pp[po];
+                        pp[po] = pp[po] + P.dt * sv;
+                    }
+                    if (n3[id] >= P.i2pml[id]) {
+                        const int si = P.i2node[id] - n3[id] + 1, pidx = g.nn[id] - n3[id] - g.nnoff[id] + 1;
+                        int sj = j, sk = k, sl = l;
+                        if (id == 0) sj = si;
+                        if (id == 1) sk = si;
+                        if (id == 2) sl = si;
+                        T *pp = P.p[id + 3];
+                        const long po = slab_index<T>(P.pn[id + 3], sj, sk, sl, ic);
+                        sv = P.dn2[pidx - 1] * sv + P.dn1[pidx - 1] * pp[po];
+                        pp[po] = pp[po] + P.dt * sv;
+                    }
+                    acc = iid == 0 ? sv : acc + sv;
+                }
+                f[ic] = acc;
+            }
+            bool interior = true;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if (n3[a] < P.i1pml[a] + 1 || n3[a] > P.i2pml[a] - 1) interior = false;
+#pragma unroll
+            for (int c = 0; c < 3; c++) f[c] = interior ? f[c] + Hf[c] : P.hg_pml ? f[c] + Hv[c] : f[c];
+            // plane-local boundary conditions (sord/boundary_cond.f90:44-66)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int b = (a + 1) % 3, cc = (a + 2) % 3;
+                int t = 0;
+                if (n3[a] == P.i1bc[a]) t = P.bc1[a] % 10;
+                if (n3[a] == P.i2bc[a]) t = P.bc2[a] % 10;
+                if (t == -1) f[b] = f[cc] = T(0);
+                if (t == 1) f[a] = T(0);
+                if (t == 3) f[0] = f[1] = f[2] = T(0);
+            }
+            if (fa >= 0 && (n3[fa] == P.irup || n3[fa] == P.irup + 1)) {
+                // fault planes: the rupture kernel finishes these nodes
+#pragma unroll
+                for (int c = 0; c < 3; c++) P.a[c][o] = f[c];
+            } else {
+                T a2 = T(0), v2 = T(0), u2 = T(0);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const T a = f[c] * mrn;
+                    const T vv = vo[c] + P.dt * a;
+                    const T uu = uo[c] + P.dt * vv;
+                    P.vn[c][o] = vv;
+                    P.un[c][o] = uu;
+                    a2 = a2 + a * a;
+                    v2 = v2 + vv * vv;
+                    u2 = u2 + uu * uu;
+                }
+                if (stat_now) st_a = tmax(st_a, a2);
+                if (stat_next) st_v = tmax(st_v, v2), st_u = tmax(st_u, u2);
             }
         }
+#pragma unroll
+        for (int c = 0; c < 3; c++) uo[c] = un1[c], vo[c] = vn1[c];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) Ulo[q][c] = Uhi[q][c], Vlo[q][c] = Vhi[q][c];
+            Glo[q] = Ghi[q];
+        }
+    }
+    if (stat_now) {
+        st_a = wave_max(st_a);
+        st_w = wave_max(st_w);
+        if (lane == 0) atomic_max_bits<T>(P.stat_cur + 0, st_a), atomic_max_bits<T>(P.stat_cur + 3, st_w);
+    }
+    if (stat_next) {
+        st_v = wave_max(st_v);
+        st_u = wave_max(st_u);
+        if (lane == 0) atomic_max_bits<T>(P.stat_nxt + 1, st_v), atomic_max_bits<T>(P.stat_nxt + 2, st_u);
     }
 }
 
 template <class T>
-void launch_fused(const Params<T> &P, const PointOps<T> &ops, const int lo[3], const int hi[3], const T sp[3],
-                  int waves, int nz, void *stat, int stat_now, int stat_next, hipStream_t s) {
-    FusedP<T> Q;
-    for (int i = 0; i < 3; i++) {
-        Q.u[i] = P.u[i], Q.v[i] = P.v[i], Q.un[i] = P.un[i], Q.vn[i] = P.vn[i];
-        Q.lo[i] = lo[i], Q.hi[i] = hi[i];
-    }
-    if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;
-    Q.gam = P.gam, Q.mr = P.mr, Q.lam = P.lam, Q.mu = P.mu, Q.yy = P.yy;
-    Q.g = P.g;
-    Q.nz = nz;
-    Q.gx = T(1) / (T(4) * sp[0]);
-    Q.gy = T(1) / (T(4) * sp[1]);
-    Q.gz = T(1) / (T(4) * sp[2]);
-    Q.ax = sp[1] * sp[2] * T(0.25);
-    Q.ay = sp[0] * sp[2] * T(0.25);
-    Q.az = sp[0] * sp[1] * T(0.25);
-    Q.h1 = P.hg_on ? P.h1 : T(0);
-    Q.dth2 = P.hg_on ? P.dth2 : T(0);
-    Q.dt = P.dt;
-    Q.vol = sp[0] * sp[1] * sp[2];
-    Q.rvol = T(1) / Q.vol;
-    Q.stat = stat;
-    Q.stat_now = stat_now;
-    Q.stat_next = stat_next;
-    Q.nbx = (hi[0] - lo[0] + 62) / 62;
-    Q.nby = (hi[1] - lo[1] + waves - 2) / (waves - 2);
-    Q.nbz = (hi[2] - lo[2] + nz) / nz;
-    const int nblk = Q.nbx * Q.nby * Q.nbz;
-    switch (waves) {
-    case 4: hipLaunchKernelGGL((k_fused<T, 4>), dim3(nblk), dim3(64, 4), 0, s, Q, ops); break;
-    case 8: hipLaunchKernelGGL((k_fused<T, 8>), dim3(nblk), dim3(64, 8), 0, s, Q, ops); break;
-    case 12: hipLaunchKernelGGL((k_fused<T, 12>), dim3(nblk), dim3(64, 12), 0, s, Q, ops); break;
-    default: hipLaunchKernelGGL((k_fused<T, 16>), dim3(nblk), dim3(64, 16), 0, s, Q, ops); break;
+void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int pml, const T sp[3],
+                  int waves, hipStream_t s) {
+    if (nwork <= 0) return;
+    FusedC<T> C;
+    C.work = work;
+    C.gx = T(1) / (T(4) * sp[0]);
+    C.gy = T(1) / (T(4) * sp[1]);
+    C.gz = T(1) / (T(4) * sp[2]);
+    C.ax = sp[1] * sp[2] * T(0.25);
+    C.ay = sp[0] * sp[2] * T(0.25);
+    C.az = sp[0] * sp[1] * T(0.25);
+    C.h1 = P.hg_on ? P.h1 : T(0);
+    C.dth2 = P.hg_on ? P.dth2 : T(0);
+    C.vol = sp[0] * sp[1] * sp[2];
+    C.rvol = T(1) / C.vol;
+    if (pml) {
+        switch (waves) {
+        // 28 LDS planes per wave: 16 waves would not fit in FP64
+        case 4: hipLaunchKernelGGL((k_fusedp<T, 4>), dim3(nwork), dim3(64, 4), 0, s, P, C, ops); break;
+        default: hipLaunchKernelGGL((k_fusedp<T, 8>), dim3(nwork), dim3(64, 8), 0, s, P, C, ops); break;
+        }
+    } else {
+        switch (waves) {
+        case 4: hipLaunchKernelGGL((k_fused<T, 4>), dim3(nwork), dim3(64, 4), 0, s, P, C, ops); break;
+        case 8: hipLaunchKernelGGL((k_fused<T, 8>), dim3(nwork), dim3(64, 8), 0, s, P, C, ops); break;
+        default: hipLaunchKernelGGL((k_fused<T, 16>), dim3(nwork), dim3(64, 16), 0, s, P, C, ops); break;
+        }
     }
 }
 
-template void launch_fused<float>(const Params<float> &, const PointOps<float> &, const int *, const int *,
-                                  const float *, int, int, void *, int, int, hipStream_t);
-template void launch_fused<double>(const Params<double> &, const PointOps<double> &, const int *, const int *,
-                                   const double *, int, int, void *, int, int, hipStream_t);
+template void launch_fused<float>(const Params<float> &, const PointOps<float> &, const Work *, int, int,
+                                  const float *, int, hipStream_t);
+template void launch_fused<double>(const Params<double> &, const PointOps<double> &, const Work *, int, int,
+                                   const double *, int, hipStream_t);
 
 }  // namespace sord

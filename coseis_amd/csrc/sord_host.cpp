@@ -836,11 +836,13 @@ extern "C" int sord_set_fused(void *h, int on) {
     });
 }
 
-extern "C" int sord_bench(void *h, int nsteps, double *ms_total, double *ms_kernel, char *kname, int name_len) {
+extern "C" int sord_bench(void *h, int nsteps, double *ms_total, double *ms_kernel, int64_t *knodes, char *kname,
+                          int name_len) {
     return dispatch(h, [&](auto &S) {
         if (!S.initialized) fail(SORD_ESTATE, "sord_bench before sord_init");
         HIPCHECK(hipSetDevice(S.device));
         S.bench(nsteps, ms_total, ms_kernel, kname, name_len);
+        *knodes = S.bench_nodes();
     });
 }
 

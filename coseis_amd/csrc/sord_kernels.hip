@@ -33,6 +33,27 @@ __constant__ signed char kHgSign[4][8] = {
 };
 
 template <class T>
+__device__ __forceinline__ void atomic_max_sq(T *slot, T v);
+template <>
+__device__ __forceinline__ void atomic_max_sq<float>(float *slot, float v) {
+    // v >= 0 or NaN: magnitude bits order like the values, NaN above +inf
+    atomicMax((unsigned int *)slot, __float_as_uint(v) & 0x7fffffffu);
+}
+template <>
+__device__ __forceinline__ void atomic_max_sq<double>(double *slot, double v) {
+    atomicMax((unsigned long long *)slot, (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull);
+}
+// maximum over the wavefront, then one atomic per wavefront
+template <class T>
+__device__ __forceinline__ void stat_max(T *slot, T v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o);
+        v = (w > v || w != w) ? w : v;
+    }
+    if ((threadIdx.x + blockDim.x * threadIdx.y) % 64 == 0) atomic_max_sq<T>(slot, v);
+}
+
+template <class T>
 __device__ __forceinline__ long slab_at(const int *n, int i, int j, int k, int comp) {
     return (long)(i - 1) + (long)n[0] * ((long)(j - 1) + (long)n[1] * (long)(k - 1)) +
            (long)comp * n[0] * n[1] * n[2];
@@ -40,14 +61,26 @@ __device__ __forceinline__ long slab_at(const int *n, int i, int j, int k, int c
 
 // ---------------------------------------------------------------------------
 // stress: sord/stress.f90:16-238
-// mode 0: stress (updates PML memory), 1: volume strain (potency), 2: moment
+// mode 0: stress (updates PML memory), 1: volume strain (potency), 2: moment,
+// 3: stress without advancing the PML memory (the fused kernels advance it)
 // ---------------------------------------------------------------------------
+template <class T, int MODE>
+__device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, int j, int k, int l);
+
 template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_stress(Params<T> P, PointOps<T> ops, int3 lo, int3 hi) {
     const int j = lo.x + blockIdx.x * blockDim.x + threadIdx.x;
     const int k = lo.y + blockIdx.y * blockDim.y + threadIdx.y;
     const int l = lo.z + blockIdx.z;
-    if (j > hi.x || k > hi.y) return;
+    const bool in = j <= hi.x && k <= hi.y;
+    T n2 = T(0);
+    if (in) n2 = stress_cell<T, MODE>(P, ops, j, k, l);
+    if ((MODE == 0 || MODE == 3) && P.stat_now) stat_max<T>(P.stat_cur + 3, n2);
+}
+
+// one cell of the stress stage; returns the squared Frobenius norm of the stress
+template <class T, int MODE>
+__device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, int j, int k, int l) {
     const Grid<T> &g = P.g;
     const int c[3] = {j, k, l};
 
@@ -156,7 +189,7 @@ __global__ __launch_bounds__(256) void k_stress(Params<T> P, PointOps<T> ops, in
             P.sig[i][o] = w1[i];
             P.sig[i + 3][o] = w2[i];
         }
-        return;
+        return T(0);
     }
     // Hooke's law (sord/stress.f90:192-213)
     const T lm = P.lam[o], m2 = T(2) * P.mu[o];
@@ -173,7 +206,7 @@ __global__ __launch_bounds__(256) void k_stress(Params<T> P, PointOps<T> ops, in
             P.sig[i][o] = w1[i];
             P.sig[i + 3][o] = w2[i];
         }
-        return;
+        return T(0);
     }
     const T rv = P.vc[o];
 #pragma unroll
@@ -187,6 +220,7 @@ __global__ __launch_bounds__(256) void k_stress(Params<T> P, PointOps<T> ops, in
         P.sig[i][o] = w1[i];
         P.sig[i + 3][o] = w2[i];
     }
+    return (w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]) + T(2) * (w2[0] * w2[0] + w2[1] * w2[1] + w2[2] * w2[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,11 +252,24 @@ __device__ __forceinline__ void local_bc(const Params<T> &P, const int n[3], T f
 }
 
 template <class T, int MODE>
+__device__ __forceinline__ void accel_node(Params<T> &P, int j, int k, int l, T st[3]);
+
+template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_accel(Params<T> P, int3 lo, int3 hi) {
     const int j = lo.x + blockIdx.x * blockDim.x + threadIdx.x;
     const int k = lo.y + blockIdx.y * blockDim.y + threadIdx.y;
     const int l = lo.z + blockIdx.z;
-    if (j > hi.x || k > hi.y) return;
+    T st[3] = {T(0), T(0), T(0)};  // |a|^2, |v|^2, |u|^2 of the updated node
+    if (j <= hi.x && k <= hi.y) accel_node<T, MODE>(P, j, k, l, st);
+    if (MODE == 2 && P.stat_now) stat_max<T>(P.stat_cur + 0, st[0]);
+    if (MODE == 2 && P.stat_next) {
+        stat_max<T>(P.stat_nxt + 1, st[1]);
+        stat_max<T>(P.stat_nxt + 2, st[2]);
+    }
+}
+
+template <class T, int MODE>
+__device__ __forceinline__ void accel_node(Params<T> &P, int j, int k, int l, T st[3]) {
     const Grid<T> &g = P.g;
     const int n[3] = {j, k, l};
     const long o = g.at(j, k, l);
@@ -370,8 +417,12 @@ __global__ __launch_bounds__(256) void k_accel(Params<T> P, int3 lo, int3 hi) {
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const T vv = P.v[i][o] + P.dt * f[i];
+        const T uu = P.u[i][o] + P.dt * vv;
         P.vn[i][o] = vv;
-        P.un[i][o] = P.u[i][o] + P.dt * vv;
+        P.un[i][o] = uu;
+        st[0] = st[0] + f[i] * f[i];
+        st[1] = st[1] + vv * vv;
+        st[2] = st[2] + uu * uu;
     }
 }
 
@@ -685,13 +736,22 @@ __global__ void k_rupture(Params<T> P, Fault<T> F, int it, T tm, int fused) {
             sv0 = sv0 + d * d;
         }
         F.svold[fq] = sqrt(sv0);
+        T sa_m = 0, sa_p = 0, sv_m = 0, sv_p = 0, su_m = 0, su_p = 0;
         for (int i = 0; i < 3; i++) {
             const T am = fm_[i] * mrm, ap = fp_[i] * mrp;
             const T vm = P.v[i][om] + P.dt * am, vp = P.v[i][op] + P.dt * ap;
+            const T um = P.u[i][om] + P.dt * vm, up = P.u[i][op] + P.dt * vp;
             P.vn[i][om] = vm;
             P.vn[i][op] = vp;
-            P.un[i][om] = P.u[i][om] + P.dt * vm;
-            P.un[i][op] = P.u[i][op] + P.dt * vp;
+            P.un[i][om] = um;
+            P.un[i][op] = up;
+            sa_m += am * am, sa_p += ap * ap, sv_m += vm * vm, sv_p += vp * vp, su_m += um * um, su_p += up * up;
+        }
+        // statistics over the fault nodes (maxima; one atomic per node pair, the plane is small)
+        if (P.stat_now) atomic_max_sq<T>(P.stat_cur + 0, fmax(sa_m, sa_p));
+        if (P.stat_next) {
+            atomic_max_sq<T>(P.stat_nxt + 1, fmax(sv_m, sv_p));
+            atomic_max_sq<T>(P.stat_nxt + 2, fmax(su_m, su_p));
         }
     }
 }
@@ -806,6 +866,7 @@ void launch_stress(const Params<T> &P, const PointOps<T> &ops, int3 lo, int3 hi,
     if (mode == 0) hipLaunchKernelGGL((k_stress<T, 0>), grd, blk, 0, s, P, ops, lo, hi);
     if (mode == 1) hipLaunchKernelGGL((k_stress<T, 1>), grd, blk, 0, s, P, ops, lo, hi);
     if (mode == 2) hipLaunchKernelGGL((k_stress<T, 2>), grd, blk, 0, s, P, ops, lo, hi);
+    if (mode == 3) hipLaunchKernelGGL((k_stress<T, 3>), grd, blk, 0, s, P, ops, lo, hi);
 }
 
 template <class T>

@@ -23,5 +23,5 @@ template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, h
 }  // namespace sord
 
 namespace sord {
-template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const int lo[3], const int hi[3], const T sp[3], int waves, int nz, void *stat, int stat_now, int stat_next, hipStream_t s);
+template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int pml, const T sp[3], int waves, hipStream_t s);
 }

@@ -49,7 +49,7 @@ def load():
     i64p = ctypes.POINTER(ctypes.c_int64)
     lib.sord_field.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), i64p, i64p, i64p]
     lib.sord_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-                               ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_int]
+                               ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_char_p, ctypes.c_int]
     lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
     lib.sord_last_error.restype = ctypes.c_char_p
     lib.sord_last_error.argtypes = []
@@ -95,10 +95,14 @@ class Solver(object):
         return it.value, nt.value
 
     def bench(self, nsteps):
+        """Advance nsteps timed: (ms all steps, ms per dominant-kernel launch,
+        nodes per launch, kernel name)."""
         tot, ker = ctypes.c_double(), ctypes.c_double()
+        nodes = ctypes.c_int64()
         name = ctypes.create_string_buffer(64)
-        _check(_lib.sord_bench(self._h, nsteps, ctypes.byref(tot), ctypes.byref(ker), name, 64))
-        return tot.value, ker.value, name.value.decode()
+        _check(_lib.sord_bench(self._h, nsteps, ctypes.byref(tot), ctypes.byref(ker), ctypes.byref(nodes),
+                               name, 64))
+        return tot.value, ker.value, nodes.value, name.value.decode()
 
     def field(self, name):
         """(device pointer, padded shape, strides, offset) of a device array."""

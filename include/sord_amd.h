@@ -65,11 +65,13 @@ int sord_status(void *handle, int *it, int *nt);
 int sord_field(void *handle, const char *name, void **dev_ptr, int64_t shape[3], int64_t strides[3],
                int64_t *offset);
 
-/* Time `nsteps` steps with HIP events on the library's stream.  Returns the
- * wall time of all steps (ms) and the mean duration of one launch of the
- * dominant solver kernel (ms) with its name.  No field i/o may be due in the
- * timed window (caller's responsibility: use a run without per-step outputs). */
-int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, char *kernel_name, int name_len);
+/* Advance `nsteps` steps (with their field i/o) timed with HIP events on the
+ * library's stream.  Returns the elapsed time of all steps (ms), the mean
+ * duration of one launch of the dominant solver kernel (ms), the number of
+ * nodes that launch advances, and the kernel's name.  No counterpart in the
+ * reference: it is the measurement hook of the benchmark. */
+int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, int64_t *kernel_nodes,
+               char *kernel_name, int name_len);
 
 /* Select the execution path: 1 fused interior kernel where it applies
  * (default), 0 stage kernels everywhere. */

@@ -52,11 +52,17 @@ def test_hip_matches_reference(case, realsize):
 def test_fused_kernel_matches_stage_kernels(case):
     """The fused interior kernel (differences of true gradients) and the stage
     kernels (volume-integrated operators) are the same algebra: in FP64 they
-    agree to rounding; in FP32 to FP32 rounding growth."""
-    for realsize, tol in ((8, 1e-9), (4, 3e-5)):
-        a = run_case(case, 'hip', realsize, fused=False)
-        b = run_case(case, 'hip', realsize, fused=True)
-        for k in a:
-            if k.startswith('stats-') and realsize == 4:
-                continue
-            assert rel_l2(b[k], a[k]) < tol, (realsize, k, rel_l2(b[k], a[k]))
+    agree to rounding; in FP32 to the case's own FP32 rounding noise, measured
+    as the distance of the FP32 stage run from the FP64 stage run (the rupture
+    case amplifies rounding through its friction law)."""
+    a64 = run_case(case, 'hip', 8, fused=False)
+    b64 = run_case(case, 'hip', 8, fused=True)
+    for k in a64:
+        assert rel_l2(b64[k], a64[k]) < 1e-9, (8, k, rel_l2(b64[k], a64[k]))
+    a = run_case(case, 'hip', 4, fused=False)
+    b = run_case(case, 'hip', 4, fused=True)
+    for k in a:
+        if k.startswith('stats-'):
+            continue
+        bar = max(3e-5, 2 * rel_l2(a[k], a64[k]))
+        assert rel_l2(b[k], a[k]) < bar, (4, k, rel_l2(b[k], a[k]), bar)
