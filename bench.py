@@ -52,7 +52,30 @@ def parse():
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--cpu-steps', type=int, default=6)
     p.add_argument('--stage-only', action='store_true', help='stage kernels only (no fused kernel)')
+    p.add_argument('--tiles', type=int, nargs=3, default=None, metavar=('W', 'NZ', 'NZB'),
+                   help='tile shape: interior waves, planes interior/boundary')
+    p.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                   help='only copy MIB MiB device to device (calibration of the HBM traffic counters)')
     return p.parse_args()
+
+
+def copy_probe(mib, reps=5):
+    """Plain device-to-device copies of known size: under rocprofv3 --pmc their
+    FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters (bytes counted
+    per byte moved).  k_copy_words moves one 4-byte word per lane per access,
+    as the solver kernels do; the runtime's own copy (wide accesses) is run
+    beside it for comparison."""
+    import torch
+    from coseis_amd import solver
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        solver.copy_probe(x.data_ptr(), y.data_ptr(), n * 4, 0)
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_probe_bytes': n * 4, 'dispatches': reps}))
 
 
 def traffic_from_profile():
@@ -109,6 +132,9 @@ def cpu_baseline(prm, files, steps):
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print('bench.py: this build runs on one GPU; a run on %d GPUs is not provided '
               '(the multi-GPU domain decomposition is design-only)' % a.gpus, file=sys.stderr)
@@ -125,7 +151,7 @@ def main():
         configs.write_inputs(files, a.realsize)
         text = sord.stage(prm, realsize=a.realsize)[0]
         torch.cuda.init()
-        s = solver.Solver(text, realsize=a.realsize, device=0, fused=not a.stage_only)
+        s = solver.Solver(text, realsize=a.realsize, device=0, fused=not a.stage_only, tiles=a.tiles)
         if a.warmup:
             s.step(a.warmup)
         torch.cuda.synchronize()

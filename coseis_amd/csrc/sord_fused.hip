@@ -47,6 +47,24 @@ __device__ __forceinline__ void atomic_max_bits<double>(double *slot, double v) 
     atomicMax((unsigned long long *)slot, (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull);
 }
 
+// value of lane + 1 / lane - 1 of the wavefront (DPP wave_shl:1 / wave_shr:1;
+// the lane without a source reads 0); a VALU operand modifier instead of a
+// trip through the LDS crossbar
+__device__ __forceinline__ int dpp_next(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_prev(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ float lane_next(float x) { return __int_as_float(dpp_next(__float_as_int(x))); }
+__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(dpp_prev(__float_as_int(x))); }
+__device__ __forceinline__ double lane_next(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_next((int)b), hi = dpp_next((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_prev(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_prev((int)b), hi = dpp_prev((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <class T>
 __device__ __forceinline__ T wave_max(T v) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -133,7 +151,8 @@ __device__ __forceinline__ void hg_modes(T f00, T f10, T f01, T f11, T g00, T g1
 // interior tiles
 // ---------------------------------------------------------------------------
 template <class T, int W>
-__global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     __shared__ T sN[6][W][64];
     __shared__ T sC[6][W][64];
     const int lane = threadIdx.x, w = threadIdx.y;
@@ -148,19 +167,26 @@ __global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, Poin
     T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
 
     // node data: W = u + gamma v (strain), F = h1 u + dt h2 v (hourglass)
-    T Nlo[4][6], Nhi[4][6];
-    T uo[3], vo[3], un1[3], vn1[3];
+    T uo[3], vo[3];
 
-    auto load_node = [&](int l, T own6[6], T uu[3], T vv[3]) {
-        const long o = g.at(jc, kc, l);
-        const T ga = P.gam[o];
+    // raw loads of one node plane (u, v, gamma), issued one iteration before use
+    const int lmax = g.nm[2];
+    auto fetch_node = [&](int l, T uu[3], T vv[3], T &ga) {
+        const long o = g.at(jc, kc, min(l, lmax));
+#pragma unroll
+        for (int i = 0; i < 3; i++) uu[i] = P.u[i][o], vv[i] = P.v[i][o];
+        ga = P.gam[o];
+    };
+    auto node_vals = [&](const T uu[3], const T vv[3], T ga, T own6[6]) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            uu[i] = P.u[i][o];
-            vv[i] = P.v[i][o];
             own6[i] = uu[i] + ga * vv[i];
             own6[3 + i] = C.h1 * uu[i] + C.dth2 * vv[i];
         }
+    };
+    auto fetch_cell = [&](int l, T mat[4]) {  // lambda, mu, yy of cell l and 1/m of node l
+        const long o = g.at(jc, kc, min(l, lmax));
+        mat[0] = P.lam[o], mat[1] = P.mu[o], mat[2] = P.yy[o], mat[3] = P.mr[o];
     };
     auto exchange = [&](const T own6[6], T N[4][6]) {
 #pragma unroll
@@ -172,14 +198,14 @@ __global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, Poin
             const T ab = sN[q][wa][lane];
             N[0][q] = own6[q];
             N[2][q] = ab;
-            N[1][q] = __shfl_down(own6[q], 1);
-            N[3][q] = __shfl_down(ab, 1);
+            N[1][q] = lane_next(own6[q]);
+            N[3][q] = lane_next(ab);
         }
     };
-    // stress and hourglass of cell (j, k, l); E: polynomial at s_l = +1, Cc: at s_l = -1
-    auto cell = [&](int l, T E[3][4], T Cc[3][4]) {
-        const long oc = g.at(jc, kc, l);
-        const T lm = P.lam[oc], m = P.mu[oc], y = P.yy[oc];
+    // stress and hourglass of cell (j, k, l) from the corner data of planes l
+    // (Nlo) and l+1 (Nhi); E: polynomial at s_l = +1, Cc: at s_l = -1
+    auto cell_at = [&](int l, const T mat[4], const T (&Nlo)[4][6], const T (&Nhi)[4][6], T E[3][4], T Cc[3][4]) {
+        const T lm = mat[0], m = mat[1], y = mat[2];
         T G[3][3];  // G[a][c]: gradient along a of component c
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -228,23 +254,36 @@ __global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, Poin
         }
     };
 
-    T own6[6], carry[3][4], E[3][4], Cn[3][4];
-    load_node(wk.lz0 - 1, own6, uo, vo);
-    exchange(own6, Nlo);
-    load_node(wk.lz0, own6, uo, vo);
+    // The plane loop is unrolled by two with alternating register sets (A/B):
+    // the loads for plane l+2 go to the set that plane l+1 does not read, so
+    // no register holding a load in flight is ever copied (a copy would make
+    // the compiler wait for the load at the end of every iteration).
+    T own6[6], E[3][4];
+    T NA[4][6], NB[4][6];  // corner data of two planes (lower, upper in turn)
+    T CA[3][4], CB[3][4];  // s_l = -1 polynomials of two cells (carried, new in turn)
+    T uA[3], vA[3], gA, mA[4], uB[3], vB[3], gB, mB[4];  // loads: plane l+1 raw, cell l + 1/m
+    fetch_node(wk.lz0 - 1, uA, vA, gA);
+    node_vals(uA, vA, gA, own6);
+    exchange(own6, NA);
+    fetch_node(wk.lz0, uo, vo, gB);
+    fetch_cell(wk.lz0 - 1, mB);
+    node_vals(uo, vo, gB, own6);
     __syncthreads();  // every wave has read the first plane from LDS
-    exchange(own6, Nhi);
-    cell(wk.lz0 - 1, E, carry);
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int c = 0; c < 6; c++) Nlo[q][c] = Nhi[q][c];
+    exchange(own6, NB);
+    fetch_node(wk.lz0 + 1, uA, vA, gA);
+    fetch_cell(wk.lz0, mA);
+    cell_at(wk.lz0 - 1, mB, NA, NB, E, CA);
 
-    for (int l = wk.lz0; l <= wk.lz1; l++) {
-        const T mrn = P.mr[g.at(jc, kc, l)];
-        load_node(l + 1, own6, un1, vn1);
+    // one plane: cu..cm hold plane l+1 and cell l (arrived), nu..nm receive plane
+    // l+2 and cell l+1; Nlo holds plane l, Nhi receives plane l+1; carry holds
+    // cell l-1's polynomial at s_l = -1, Cn receives cell l's
+    auto plane = [&](int l, bool act, T (&cu)[3], T (&cv)[3], T &cg, T (&cm)[4], T (&nu)[3], T (&nv)[3], T &ng,
+                     T (&nm)[4], T (&Nlo)[4][6], T (&Nhi)[4][6], T (&carry)[3][4], T (&Cn)[3][4]) {
+        node_vals(cu, cv, cg, own6);
+        fetch_node(l + 2, nu, nv, ng);
+        fetch_cell(l + 1, nm);
         exchange(own6, Nhi);
-        cell(l, E, Cn);
+        cell_at(l, cm, Nlo, Nhi, E, Cn);
         T R[3][2];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -254,8 +293,6 @@ __global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, Poin
             R[c][1] = qj + qjk;
             sC[2 * c][w][lane] = q0 - qk;
             sC[2 * c + 1][w][lane] = qj - qjk;
-#pragma unroll
-            for (int q = 0; q < 4; q++) carry[c][q] = Cn[c][q];
         }
         __syncthreads();
         const int wb = w > 0 ? w - 1 : 0;
@@ -264,10 +301,11 @@ __global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, Poin
         for (int c = 0; c < 3; c++) {
             const T r0 = R[c][0] + sC[2 * c][wb][lane];
             const T rj = R[c][1] + sC[2 * c + 1][wb][lane];
-            f[c] = (r0 + rj) + __shfl_up(r0 - rj, 1);
+            f[c] = (r0 + rj) + lane_prev(r0 - rj);
         }
-        if (own) {
+        if (own && act) {
             const long o = g.at(j, k, l);
+            const T mrn = cm[3];
             T a2 = T(0), v2 = T(0), u2 = T(0);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -284,11 +322,13 @@ __global__ __launch_bounds__(64 * W) void k_fused(Params<T> P, FusedC<T> C, Poin
             if (stat_next) st_v = tmax(st_v, v2), st_u = tmax(st_u, u2);
         }
 #pragma unroll
-        for (int c = 0; c < 3; c++) uo[c] = un1[c], vo[c] = vn1[c];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int c = 0; c < 6; c++) Nlo[q][c] = Nhi[q][c];
+        for (int c = 0; c < 3; c++) uo[c] = cu[c], vo[c] = cv[c];
+    };
+    // both planes of a pair always run (one loop back-edge keeps the register
+    // sets in place); past the last plane of the item the second one stores nothing
+    for (int l = wk.lz0; l <= wk.lz1; l += 2) {
+        plane(l, true, uA, vA, gA, mA, uB, vB, gB, mB, NB, NA, CA, CB);
+        plane(l + 1, l + 1 <= wk.lz1, uB, vB, gB, mB, uA, vA, gA, mA, NA, NB, CB, CA);
     }
     if (stat_now) {
         st_a = wave_max(st_a);
@@ -311,16 +351,29 @@ __device__ __forceinline__ long slab_index(const int *n, int i, int j, int k, in
            (long)comp * n[0] * n[1] * n[2];
 }
 
-template <class T, int W>
+// MASK bit a: the item has real cells of a PML slab of axis a (else no cell or
+// node of the item is damped along a, and that axis's PML code folds away).
+// Rows: an item of width je-jb+1 runs LW = width+2 lanes per node row (Work.pad0)
+// and R = 64/LW rows per wavefront, so narrow items (the x slabs) fill the
+// wavefront; node rows of the block are w*R + lane/LW, with halo rows 0 and W*R-1.
+template <class T, int W, int MASK>
 __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
-    __shared__ T sN[7][W][64];
-    __shared__ T sC[21][W][64];
+    constexpr int NS = W * 64 + 64;  // LDS slots per value; lanes past R*LW use the last 64
+    __shared__ T sN[7][NS];
+    __shared__ T sC[21][NS];
     const int lane = threadIdx.x, w = threadIdx.y;
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;  // uniform over the block
-    const int j = wk.jb - 1 + lane, k = wk.kb - 1 + w;
-    const bool ownjk = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= wk.je && k <= wk.ke;
-    const bool cellv = lane <= 62 && w <= W - 2 && j <= wk.je && k <= wk.ke;
+    const int LW = wk.pad0, R = 64 / LW, NR = W * R;
+    const int jj = lane % LW, rr = lane / LW;
+    const bool valid = rr < R;
+    const int row = w * R + rr;
+    const int t = valid ? w * R * LW + lane : W * 64 + lane;  // LDS slot
+    const int tab = valid && row + 1 < NR ? t + LW : t;     // row above
+    const int tbe = valid && row > 0 ? t - LW : t;           // row below
+    const int j = wk.jb - 1 + jj, k = wk.kb - 1 + row;
+    const bool ownjk = valid && jj >= 1 && jj <= LW - 2 && row >= 1 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
+    const bool cellv = valid && jj <= LW - 2 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
     const Grid<T> &g = P.g;
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
     const bool stat_now = P.stat_now, stat_next = P.stat_next;
@@ -344,17 +397,16 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
     };
     auto exchange = [&](const T own7[7], T U[4][3], T V[4][3], T G[4]) {
 #pragma unroll
-        for (int q = 0; q < 7; q++) sN[q][w][lane] = own7[q];
+        for (int q = 0; q < 7; q++) sN[q][t] = own7[q];
         __syncthreads();
-        const int wa = w + 1 < W ? w + 1 : w;
         T x[4][7];
 #pragma unroll
         for (int q = 0; q < 7; q++) {
-            const T ab = sN[q][wa][lane];
+            const T ab = sN[q][tab];
             x[0][q] = own7[q];
             x[2][q] = ab;
-            x[1][q] = __shfl_down(own7[q], 1);
-            x[3][q] = __shfl_down(ab, 1);
+            x[1][q] = lane_next(own7[q]);
+            x[3][q] = lane_next(ab);
         }
 #pragma unroll
         for (int c = 0; c < 4; c++) {
@@ -383,8 +435,8 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
         bool inLo[3], inHi[3], inS[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            inLo[a] = c3[a] <= P.i1pml[a];
-            inHi[a] = c3[a] >= P.i2pml[a] - 1;
+            inLo[a] = (MASK >> a & 1) && c3[a] <= P.i1pml[a];
+            inHi[a] = (MASK >> a & 1) && c3[a] >= P.i2pml[a] - 1;
             inS[a] = inLo[a] || inHi[a];
         }
         const T gsp[3] = {C.gx, C.gy, C.gz};
@@ -466,7 +518,12 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
                         C.h1 * Ulo[2][c] + C.dth2 * Vlo[2][c], C.h1 * Ulo[3][c] + C.dth2 * Vlo[3][c],
                         C.h1 * Uhi[0][c] + C.dth2 * Vhi[0][c], C.h1 * Uhi[1][c] + C.dth2 * Vhi[1][c],
                         C.h1 * Uhi[2][c] + C.dth2 * Vhi[2][c], C.h1 * Uhi[3][c] + C.dth2 * Vhi[3][c], h);
-            hg_modes<T>(Vlo[0][c], Vlo[1][c], Vlo[2][c], Vlo[3][c], Vhi[0][c], Vhi[1][c], Vhi[2][c], Vhi[3][c], hv);
+            if (MASK) {  // viscous hourglass of PML nodes (none without damped cells)
+                hg_modes<T>(Vlo[0][c], Vlo[1][c], Vlo[2][c], Vlo[3][c], Vhi[0][c], Vhi[1][c], Vhi[2][c], Vhi[3][c], hv);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) hv[q] = T(0);
+            }
 #pragma unroll
             for (int q = 0; q < 4; q++) h[q] = yf * h[q], hv[q] = yv * hv[q];
             E[c][0] = A[c][0];
@@ -508,7 +565,7 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
         for (int c = 0; c < 3; c++) {
             T q[9];
 #pragma unroll
-            for (int t = 0; t < 9; t++) q[t] = E[c][t] + carry[c][t];
+            for (int z = 0; z < 9; z++) q[z] = E[c][z] + carry[c][z];
             R[c][0] = q[0];
             R[c][1] = q[1];
             R[c][2] = q[2];
@@ -516,30 +573,29 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
             R[c][4] = q[3] + q[5];   // s_j coefficient
             R[c][5] = q[7];
             R[c][6] = q[6] + q[8];
-            sC[7 * c + 0][w][lane] = q[0];
-            sC[7 * c + 1][w][lane] = -q[1];
-            sC[7 * c + 2][w][lane] = q[2];
-            sC[7 * c + 3][w][lane] = -q[4];
-            sC[7 * c + 4][w][lane] = q[3] - q[5];
-            sC[7 * c + 5][w][lane] = -q[7];
-            sC[7 * c + 6][w][lane] = q[6] - q[8];
+            sC[7 * c + 0][t] = q[0];
+            sC[7 * c + 1][t] = -q[1];
+            sC[7 * c + 2][t] = q[2];
+            sC[7 * c + 3][t] = -q[4];
+            sC[7 * c + 4][t] = q[3] - q[5];
+            sC[7 * c + 5][t] = -q[7];
+            sC[7 * c + 6][t] = q[6] - q[8];
 #pragma unroll
             for (int t = 0; t < 9; t++) carry[c][t] = Cn[c][t];
         }
         __syncthreads();
-        const int wb = w > 0 ? w - 1 : 0;
         // j stage: divergences per direction and the two hourglass variants
         T D[3][3], Hf[3], Hv[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             T r[7];
 #pragma unroll
-            for (int t = 0; t < 7; t++) r[t] = R[c][t] + sC[7 * c + t][wb][lane];
-            D[c][0] = r[0] + __shfl_up(-r[0], 1);
-            D[c][1] = r[1] + __shfl_up(r[1], 1);
-            D[c][2] = r[2] + __shfl_up(r[2], 1);
-            Hf[c] = (r[3] + r[4]) + __shfl_up(r[3] - r[4], 1);
-            Hv[c] = (r[5] + r[6]) + __shfl_up(r[5] - r[6], 1);
+            for (int q = 0; q < 7; q++) r[q] = R[c][q] + sC[7 * c + q][tbe];
+            D[c][0] = r[0] + lane_prev(-r[0]);
+            D[c][1] = r[1] + lane_prev(r[1]);
+            D[c][2] = r[2] + lane_prev(r[2]);
+            Hf[c] = (r[3] + r[4]) + lane_prev(r[3] - r[4]);
+            Hv[c] = (r[5] + r[6]) + lane_prev(r[5] - r[6]);
         }
         const int n3[3] = {j, k, l};
         bool own = ownjk;
@@ -557,7 +613,7 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
                     const int id = (ic + iid) % 3;
                     T sv = D[ic][id];
                     // PML momentum memory (sord/acceleration.f90:295-362)
-                    if (n3[id] <= P.i1pml[id]) {
+                    if ((MASK >> id & 1) && n3[id] <= P.i1pml[id]) {
                         const int si = n3[id] - P.i1node[id] + 1, pidx = n3[id] + g.nnoff[id];
                         int sj = j, sk = k, sl = l;
                         if (id == 0) sj = si;
@@ -568,7 +624,7 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
                         sv = P.dn2[pidx - 1] * sv + P.dn1[pidx - 1] * pp[po];
                         pp[po] = pp[po] + P.dt * sv;
                     }
-                    if (n3[id] >= P.i2pml[id]) {
+                    if ((MASK >> id & 1) && n3[id] >= P.i2pml[id]) {
                         const int si = P.i2node[id] - n3[id] + 1, pidx = g.nn[id] - n3[id] - g.nnoff[id] + 1;
                         int sj = j, sk = k, sl = l;
                         if (id == 0) sj = si;
@@ -586,9 +642,9 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
             bool interior = true;
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                if (n3[a] < P.i1pml[a] + 1 || n3[a] > P.i2pml[a] - 1) interior = false;
+                if ((MASK >> a & 1) && (n3[a] < P.i1pml[a] + 1 || n3[a] > P.i2pml[a] - 1)) interior = false;
 #pragma unroll
-            for (int c = 0; c < 3; c++) f[c] = interior ? f[c] + Hf[c] : P.hg_pml ? f[c] + Hv[c] : f[c];
+            for (int c = 0; c < 3; c++) f[c] = interior ? f[c] + Hf[c] : (MASK && P.hg_pml) ? f[c] + Hv[c] : f[c];
             // plane-local boundary conditions (sord/boundary_cond.f90:44-66)
 #pragma unroll
             for (int a = 0; a < 3; a++) {
@@ -642,8 +698,13 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
     }
 }
 
+template <class T, int M>
+static void launch_boundary(int nwork, hipStream_t s, const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops) {
+    hipLaunchKernelGGL((k_fusedp<T, 8, M>), dim3(nwork), dim3(64, 8), 0, s, P, C, ops);
+}
+
 template <class T>
-void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int pml, const T sp[3],
+void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3],
                   int waves, hipStream_t s) {
     if (nwork <= 0) return;
     FusedC<T> C;
@@ -658,11 +719,16 @@ void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, 
     C.dth2 = P.hg_on ? P.dth2 : T(0);
     C.vol = sp[0] * sp[1] * sp[2];
     C.rvol = T(1) / C.vol;
-    if (pml) {
-        switch (waves) {
-        // 28 LDS planes per wave: 16 waves would not fit in FP64
-        case 4: hipLaunchKernelGGL((k_fusedp<T, 4>), dim3(nwork), dim3(64, 4), 0, s, P, C, ops); break;
-        default: hipLaunchKernelGGL((k_fusedp<T, 8>), dim3(nwork), dim3(64, 8), 0, s, P, C, ops); break;
+    if (mask >= 0) {  // boundary kernel (8 waves), specialised on the damped axes
+        switch (mask) {
+        case 0: launch_boundary<T, 0>(nwork, s, P, C, ops); break;
+        case 1: launch_boundary<T, 1>(nwork, s, P, C, ops); break;
+        case 2: launch_boundary<T, 2>(nwork, s, P, C, ops); break;
+        case 3: launch_boundary<T, 3>(nwork, s, P, C, ops); break;
+        case 4: launch_boundary<T, 4>(nwork, s, P, C, ops); break;
+        case 5: launch_boundary<T, 5>(nwork, s, P, C, ops); break;
+        case 6: launch_boundary<T, 6>(nwork, s, P, C, ops); break;
+        default: launch_boundary<T, 7>(nwork, s, P, C, ops); break;
         }
     } else {
         switch (waves) {

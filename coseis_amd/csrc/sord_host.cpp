@@ -228,6 +228,9 @@ struct Solver {
     // device
     int device = 0;
     hipStream_t stream = nullptr;
+    static const int NSIDE = 3;            // side streams of the boundary kernels (default priority)
+    hipStream_t side[NSIDE] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[NSIDE] = {};
     std::vector<void *> dallocs;
     Params<T> P;
     Fault<T> F;
@@ -251,7 +254,13 @@ struct Solver {
     }
 
     ~Solver() {
+        if (stream) (void)hipStreamSynchronize(stream);  // the side streams join the main stream every step
         for (void *p : dallocs) (void)hipFree(p);
+        for (int i = 0; i < NSIDE; i++) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -836,6 +845,21 @@ extern "C" int sord_set_fused(void *h, int on) {
     });
 }
 
+extern "C" int sord_set_tiles(void *h, int waves, int planes, int planes_boundary) {
+    return dispatch(h, [&](auto &S) {
+        if ((waves != 4 && waves != 8 && waves != 16) || planes < 1 || planes > 4096 || planes_boundary < 1 ||
+            planes_boundary > 4096)
+            fail(SORD_EINPUT, "tile shape: waves 4, 8 or 16, planes 1..4096");
+        S.tile_waves = waves;
+        S.tile_nz = planes, S.tile_nz_b = planes_boundary;
+        if (S.initialized) {
+            HIPCHECK(hipSetDevice(S.device));
+            HIPCHECK(hipStreamSynchronize(S.stream));
+            S.plan_regions();
+        }
+    });
+}
+
 extern "C" int sord_bench(void *h, int nsteps, double *ms_total, double *ms_kernel, int64_t *knodes, char *kname,
                           int name_len) {
     return dispatch(h, [&](auto &S) {
@@ -843,6 +867,15 @@ extern "C" int sord_bench(void *h, int nsteps, double *ms_total, double *ms_kern
         HIPCHECK(hipSetDevice(S.device));
         S.bench(nsteps, ms_total, ms_kernel, kname, name_len);
         *knodes = S.bench_nodes();
+    });
+}
+
+extern "C" int sord_copy_probe(const void *src, void *dst, int64_t nbytes, int device) {
+    return guard([&]() {
+        if (nbytes < 0 || nbytes % 4) fail(SORD_EINPUT, "copy probe: a whole number of 4-byte words");
+        HIPCHECK(hipSetDevice(device));
+        sord::launch_copy_words(src, dst, (long)(nbytes / 4), nullptr);
+        HIPCHECK(hipDeviceSynchronize());
     });
 }
 

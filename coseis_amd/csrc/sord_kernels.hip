@@ -1007,4 +1007,19 @@ void launch_fault_sa(const Params<T> &P, const Fault<T> &F, hipStream_t s) {
 INST(float)
 INST(double)
 
+// ---------------------------------------------------------------------------
+// calibration copy: one 4-byte word per lane per access, the access width of
+// the solver kernels (the HBM counters count 128-byte requests differently)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy_words(const unsigned *src, unsigned *dst, long n) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        dst[q] = src[q];
+}
+void launch_copy_words(const void *src, void *dst, long nwords, hipStream_t s) {
+    if (nwords <= 0) return;
+    const long blocks = std::min<long>((nwords + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)blocks), dim3(256), 0, s, (const unsigned *)src, (unsigned *)dst,
+                       nwords);
+}
+
 }  // namespace sord

@@ -14,6 +14,7 @@ template <class T> void launch_plane_copy(T *f, View g, const int n[3], int ax, 
 template <class T> void launch_sample(const T *f, View g, const Samples<T> &smp, T *out, hipStream_t s);
 template <class T> void launch_point_add(T *f, View g, const int lo[3], const int hi[3], const T xi[3], T val, hipStream_t s);
 template <class T> void launch_norm(T *out, T *const a[3], T *const b[3], const Grid<T> &g, const int lo[3], const int hi[3], const int st[3], const int n[3], hipStream_t s);
+void launch_copy_words(const void *src, void *dst, long nwords, hipStream_t s);
 template <class T> int launch_maxnorm(T *const a[3], T *const b[3], const Grid<T> &g, const int lo[3], const int hi[3], T *pval, long *ploc, hipStream_t s);
 template <class T> void launch_vbc_plane(const Params<T> &P, int ax, int dst, int src, int mask, const T sg[3], int kind, hipStream_t s);
 template <class T> void launch_rupture(const Params<T> &P, const Fault<T> &F, int it, T tm, int fused, hipStream_t s);
@@ -23,5 +24,7 @@ template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, h
 }  // namespace sord
 
 namespace sord {
-template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int pml, const T sp[3], int waves, hipStream_t s);
+// mask -1: interior kernel with `waves` waves per block; 0..7: boundary kernel
+// specialised on the damped axes (bit a: axis a), 8 waves per block
+template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, hipStream_t s);
 }

@@ -17,7 +17,7 @@ ERRORS = {
 }
 EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field',
-    'sord_bench', 'sord_set_fused', 'sord_last_error', 'sord_destroy',
+    'sord_bench', 'sord_set_fused', 'sord_set_tiles', 'sord_copy_probe', 'sord_last_error', 'sord_destroy',
 ]
 
 _lib = None
@@ -51,11 +51,13 @@ def load():
     lib.sord_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_char_p, ctypes.c_int]
     lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
+    lib.sord_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.sord_copy_probe.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     lib.sord_last_error.restype = ctypes.c_char_p
     lib.sord_last_error.argtypes = []
     lib.sord_destroy.argtypes = [vp]
     lib.sord_destroy.restype = None
-    for name in EXPORTS[:-2] + ['sord_set_fused']:
+    for name in EXPORTS[:-2]:
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -67,6 +69,12 @@ def _check(rc):
         raise SolverError(rc, msg)
 
 
+def copy_probe(src_ptr, dst_ptr, nbytes, device=0):
+    """Copy nbytes between two device buffers with the library's calibration
+    kernel (one 4-byte word per lane per access) and wait for it."""
+    _check(load().sord_copy_probe(src_ptr, dst_ptr, nbytes, device))
+
+
 class Solver(object):
     """One solver handle: ``sord.in`` text in, reference output files out.
 
@@ -74,14 +82,20 @@ class Solver(object):
     output files, as the reference ``sord.x`` writes them into its cwd.
     """
 
-    def __init__(self, text, realsize=4, device=0, fused=True):
+    def __init__(self, text, realsize=4, device=0, fused=True, tiles=None):
         lib = load()
         self._h = ctypes.c_void_p()
         _check(lib.sord_create(text.encode(), realsize, device, ctypes.byref(self._h)))
         self.realsize = realsize
         if not fused:
             _check(lib.sord_set_fused(self._h, 0))
+        if tiles:
+            self.set_tiles(*tiles)
         _check(lib.sord_init(self._h))
+
+    def set_tiles(self, waves=8, planes=16, planes_boundary=16):
+        """Tile shape of the fused kernels (performance only; see sord_set_tiles)."""
+        _check(_lib.sord_set_tiles(self._h, waves, planes, planes_boundary))
 
     def step(self, n=1):
         _check(_lib.sord_step(self._h, n))

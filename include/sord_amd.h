@@ -77,6 +77,19 @@ int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, in
  * (default), 0 stage kernels everywhere. */
 int sord_set_fused(void *handle, int on);
 
+/* Tile shape of the fused kernels: waves per block of the interior kernel
+ * (4, 8 or 16; a block advances waves-2 node rows) and the planes one block
+ * marches in the interior and boundary kernels.  Takes effect at the next
+ * step; the results do not depend on it.  No counterpart in the reference
+ * (its loops have no tiles): the hook of the autotuner. */
+int sord_set_tiles(void *handle, int waves, int planes, int planes_boundary);
+
+/* Copy nbytes (a multiple of 4) between two device buffers of `device` with a
+ * plain kernel that moves one 4-byte word per lane per access, and wait for
+ * it.  No counterpart in the reference: the calibration of the HBM traffic
+ * counters (FETCH_SIZE, WRITE_SIZE) for the access width of the solver. */
+int sord_copy_probe(const void *src, void *dst, int64_t nbytes, int device);
+
 /* Message of the last error on this thread ("" if none). */
 const char *sord_last_error(void);
 
