@@ -15,9 +15,13 @@ over the K steps (nodes nx*ny*nz per step, halo excluded).
 roofline: the fused interior kernel (k_fused) is the dominant kernel; its
 algorithmic bytes per node-update are 68 B in FP32 (read u, v: 24 B, write u,
 v: 24 B, cell lambda, mu, yy: 12 B, node 1/m, gamma: 8 B); one launch advances
-the interior box, so achieved = 68 B x nodes / mean launch duration.  traffic:
-HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, read from the
-committed summary in profiles/ when it exists.
+the interior box, so achieved = 68 B x nodes / mean launch duration (HIP
+events on its stream; the boundary kernels run beside it on side streams, so
+the duration includes that sharing).  traffic: HBM bytes per launch from
+rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy with the
+same 4-byte access width), read from the committed summary in profiles/.
+step_roofline: the whole step, 68 B per node plus 48 B per node and damped
+axis (PML memory g and p), over the measured step time.
 
 cpu_baseline: the reference solver itself (the amdflang -O3 -fopenmp build of
 the reference Fortran, oracle/_ref/r4/sord.x) on this machine's host cores, on
@@ -39,6 +43,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 ALG_BYTES = {4: 68, 8: 136}
+PML_BYTES = {4: 48, 8: 96}  # per node and damped axis: read + write of g and p (3 components each)
+
+
+def slab_memberships(prm):
+    """Sum over nodes of the number of axes along which the node is damped
+    (npml planes on each PML face)."""
+    n = prm['shape'][:3]
+    npml = prm.get('npml', 10)
+    tot = 0
+    for a in range(3):
+        planes = npml * ((prm['bc1'][a] == 'pml') + (prm['bc2'][a] == 'pml'))
+        tot += planes * n[(a + 1) % 3] * n[(a + 2) % 3]
+    return tot
 HBM_PEAK_GBS = 8000.0
 PROFILE = os.path.join(ROOT, 'profiles', 'k_fused_layered321_traffic.txt')
 
@@ -167,6 +184,7 @@ def main():
     secs = ms_total / 1e3
     value = nodes * a.steps / secs / 1e9
     alg = ALG_BYTES[a.realsize] * knodes
+    step_bytes = ALG_BYTES[a.realsize] * nodes + PML_BYTES[a.realsize] * slab_memberships(prm)
     achieved = alg / (ms_kernel / 1e3) / 1e9 if ms_kernel > 0 else None
     line = {
         'metric': 'Gnode-updates/s', 'value': round(value, 3), 'unit': 'Gnode-updates/s',
@@ -177,7 +195,8 @@ def main():
         'config': {'workload': 'layered half-space 321x321x161, dx 50 m, dt 0.004 s, free surface + '
                                '10-node PML, viscous hourglass, double couple, 16x16 receivers',
                    'nodes': nodes, 'realsize': a.realsize,
-                   'path': 'stage kernels' if a.stage_only else 'fused interior + stage shell'},
+                   'path': 'stage kernels' if a.stage_only else
+                           'fused interior kernel + fused boundary kernels on side streams'},
         'wall_s': round(wall, 4),
         'roofline': {
             'bound': 'hbm', 'kernel': kname,
@@ -187,7 +206,11 @@ def main():
             'alg_bytes_per_launch': alg, 'nodes_per_launch': knodes,
             'launch_ms': round(ms_kernel, 5),
         },
-        'hbm_frac_of_step': round(ALG_BYTES[a.realsize] * nodes * a.steps / secs / 1e9 / HBM_PEAK_GBS, 4),
+        'step_roofline': {
+            'alg_bytes_per_step': step_bytes,
+            'achieved': round(step_bytes * a.steps / secs / 1e9, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+            'frac': round(step_bytes * a.steps / secs / 1e9 / HBM_PEAK_GBS, 4),
+        },
         'batched_vs_plain': None,
         'grad_over_forward': None,
     }
