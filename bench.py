@@ -71,6 +71,9 @@ def parse():
     p.add_argument('--stage-only', action='store_true', help='stage kernels only (no fused kernel)')
     p.add_argument('--tiles', type=int, nargs=3, default=None, metavar=('W', 'NZ', 'NZB'),
                    help='tile shape: interior waves, planes interior/boundary')
+    p.add_argument('--no-batch', action='store_true', help='skip the batched-ensemble comparison')
+    p.add_argument('--batch-scenarios', type=int, default=64)
+    p.add_argument('--batch-steps', type=int, default=100)
     p.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                    help='only copy MIB MiB device to device (calibration of the HBM traffic counters)')
     return p.parse_args()
@@ -147,6 +150,59 @@ def cpu_baseline(prm, files, steps):
     }
 
 
+def batched_leg(nscen, steps, warmup=5, tune_steps=10):
+    """Batched ensemble against sequential plain runs of the same scenarios.
+
+    The ensemble (coseis_amd.configs.ensemble): nscen seeded scenarios with
+    edges log-uniform in 48..160 nodes, dx = 100 m, dt = 0.008 s, 1-4 layers,
+    a double couple each, 10-node PML, FP32.  Batched: one solver.Batch; after
+    warmup steps the autotuner picks the planes per grid-size bucket (its
+    trials advance the batch), then `steps` steps are timed (HIP events).
+    Plain: each scenario in its own solver.Solver, warmup steps, then the same
+    `steps` timed; the times add up.  Both exclude initialisation."""
+    from coseis_amd import autotune, configs, solver, sord
+    ens = configs.ensemble(n=nscen, nt=warmup + tune_steps * 14 + steps + 8)
+    shapes = [p['shape'] for p, _ in ens]
+    base = tempfile.mkdtemp(prefix='sord-ens-')
+    cwd = os.getcwd()
+    try:
+        texts, dirs = [], []
+        for q, (prm, files) in enumerate(ens):
+            d = os.path.join(base, 'b%02d' % q)
+            os.makedirs(d)
+            os.chdir(d)
+            configs.write_inputs(files)
+            texts.append(sord.stage(prm)[0])
+            dirs.append(d)
+        os.chdir(cwd)
+        with solver.Batch(texts, dirs) as b:
+            b.step(warmup)
+            choice, _, trials = autotune.tune_batch(b, shapes, steps=tune_steps)
+            ms_b, nu_b = b.bench(steps)
+        ms_p, nu_p = 0.0, 0
+        for q, text in enumerate(texts):
+            d = os.path.join(base, 'p%02d' % q)
+            os.makedirs(d)
+            os.chdir(d)
+            with solver.Solver(text) as s:
+                s.step(warmup)
+                ms, _, _, _ = s.bench(steps)
+            os.chdir(cwd)
+            ms_p += ms
+            nu_p += shapes[q][0] * shapes[q][1] * shapes[q][2] * steps
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(base)
+    rb, rp = nu_b / ms_b / 1e6, nu_p / ms_p / 1e6
+    return round(rb / rp, 3), {
+        'scenarios': nscen, 'steps': steps, 'nodes': sum(s[0] * s[1] * s[2] for s in shapes),
+        'batched_gnode_s': round(rb, 3), 'batched_ms_per_step': round(ms_b / steps, 4),
+        'plain_sequential_gnode_s': round(rp, 3), 'plain_ms_per_step_sum': round(ms_p / steps, 4),
+        'tiles_by_bucket': {str(k): list(v) for k, v in sorted(choice.items())},
+        'autotune_trials': len(trials),
+    }
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -214,6 +270,8 @@ def main():
         'batched_vs_plain': None,
         'grad_over_forward': None,
     }
+    if not a.no_batch:
+        line['batched_vs_plain'], line['batched'] = batched_leg(a.batch_scenarios, a.batch_steps)
     if not a.no_cpu:
         line['cpu_baseline'] = cpu_baseline(prm, files, a.cpu_steps)
     print(json.dumps(line))

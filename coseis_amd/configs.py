@@ -83,6 +83,54 @@ def rupture_400(nt=1500):
     return prm, files
 
 
+def ensemble(n=64, seed=0, nt=800):
+    """Batched ensemble: n scenarios with seeded uneven meshes.  Edges per
+    axis drawn log-uniformly from 48 to 160 nodes, dx = 100 m, dt = 0.008 s,
+    10-node PML on all faces.  Each model is a stack of 1 to 4 layers with vs
+    uniform in [1500, 3500] m/s (increasing with depth), vp = 1.732 vs and
+    Gardner density rho = 310 vp^0.25; one double couple (random fault normal
+    and slip direction) with M0 log-uniform in [1e15, 1e18] N m and a Brune
+    pulse, tau uniform in [0.1, 0.3] s, at a random node at least 4 nodes
+    inside the PML.  Output: vz of the top interior plane at the last step.
+    Returns a list of (prm, files)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        shape = [int(round(float(np.exp(rng.uniform(np.log(48), np.log(160)))))) for _ in range(3)]
+        nz = shape[2]
+        nl = int(rng.integers(1, 5))
+        vs = np.sort(rng.uniform(1500.0, 3500.0, nl))
+        cuts = np.sort(rng.choice(np.arange(1, nz - 2), size=nl - 1, replace=False)) if nl > 1 else []
+        vp = 1.732 * vs
+        rho = 310.0 * vp ** 0.25
+        # layer 0 on top (l = 0), the deepest layer fills the rest
+        prm = {
+            'shape': shape + [nt], 'delta': [100.0, 100.0, 100.0, 0.008],
+            'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 10,
+            'rho': [float(rho[-1])], 'vp': [float(vp[-1])], 'vs': [float(vs[-1])], 'gam': [0.0],
+        }
+        top = 0
+        for i, c in enumerate(cuts):
+            zone = ([[], [], [int(top), int(c)]], '=')
+            prm['rho'].append(zone + (float(rho[i]),))
+            prm['vp'].append(zone + (float(vp[i]),))
+            prm['vs'].append(zone + (float(vs[i]),))
+            top = c
+        m0 = 10.0 ** rng.uniform(15.0, 18.0)
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        slip = np.cross(nrm, rng.normal(size=3))
+        slip /= np.linalg.norm(slip)
+        x = [float(rng.integers(14, e - 14)) for e in shape]
+        tau = float(rng.uniform(0.1, 0.3))
+        comps = {'mxx': (0, 0), 'myy': (1, 1), 'mzz': (2, 2), 'myz': (1, 2), 'mzx': (2, 0), 'mxy': (0, 1)}
+        for f, (a, b) in comps.items():
+            prm[f] = [(x + [[]], '.', float(m0 * (slip[a] * nrm[b] + slip[b] * nrm[a])), 'brune', tau)]
+        prm['vz'] = [([[], [], 10, -1], '=>', 'vz-top.bin')]
+        out.append((prm, {}))
+    return out
+
+
 def write_inputs(files, realsize=4):
     for name, a in files.items():
         np.asarray(a, dtype='<f%d' % realsize).T.tofile(name)

@@ -79,6 +79,32 @@ struct Work {
     int jb, je, kb, ke, lz0, lz1, pad0, pad1;
 };
 
+// constants of the fused kernels (uniform spacing per axis)
+template <class T>
+struct FusedC {
+    const Work *work;
+    T gx, gy, gz;  // 1 / (4 dx_a): gradient from the sum of four edge differences
+    T ax, ay, az;  // dx_b dx_c / 4: divergence weights
+    T h1, dth2;    // hourglass(1), dt * hourglass(2)
+    T vol, rvol;   // cell volume and its inverse
+};
+
+// statistics slots of one scenario of a batch to clear before a step
+template <class T>
+struct StatZero {
+    T *cur, *nxt;   // slot sets of this step and of the next
+    int now, next;  // this step computes statistics / the next one does
+};
+
+// one scenario of a batch: what the plain fused kernels get as arguments
+template <class T>
+struct Scen {
+    Params<T> P;
+    FusedC<T> C;
+    PointOps<T> ops;
+    int active;  // 0: the scenario sits this step out (its items return at once)
+};
+
 struct Box {
     int lo[3], hi[3], st[3], n[3];  // 1-based local indices, steps, counts
 };

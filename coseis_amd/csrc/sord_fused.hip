@@ -78,14 +78,6 @@ __device__ __forceinline__ T tmax(T a, T b) {
     return (b > a || b != b) ? b : a;
 }
 
-template <class T>
-struct FusedC {
-    const Work *work;
-    T gx, gy, gz;  // 1 / (4 dx_a): gradient from the sum of four edge differences
-    T ax, ay, az;  // dx_b dx_c / 4: divergence weights
-    T h1, dth2;    // hourglass(1), dt * hourglass(2)
-    T vol, rvol;   // cell volume and its inverse
-};
 
 // ---------------------------------------------------------------------------
 // shared pieces
@@ -151,13 +143,11 @@ __device__ __forceinline__ void hg_modes(T f00, T f10, T f01, T f11, T g00, T g1
 // interior tiles
 // ---------------------------------------------------------------------------
 template <class T, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
-void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+__device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
+                                              const Work &wk) {
     __shared__ T sN[6][W][64];
     __shared__ T sC[6][W][64];
     const int lane = threadIdx.x, w = threadIdx.y;
-    const Work wk = C.work[blockIdx.x];
-    if (wk.lz0 > wk.lz1) return;  // uniform over the block
     const int j = wk.jb - 1 + lane, k = wk.kb - 1 + w;
     const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= wk.je && k <= wk.ke;
     const bool cellv = lane <= 62 && w <= W - 2 && j <= wk.je && k <= wk.ke;
@@ -342,6 +332,25 @@ void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     }
 }
 
+// one work item per block; the plain kernel gets one scenario in its
+// arguments, the batch kernel looks its item's scenario up (Work.pad1)
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+    const Work wk = C.work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;  // uniform over the block
+    interior_item<T, W>(P, C, ops, wk);
+}
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_fused_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work) {
+    const Work wk = work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;
+    const Scen<T> &S = sc[wk.pad1];
+    if (!S.active) return;  // uniform over the block
+    interior_item<T, W>(S.P, S.C, S.ops, wk);
+}
+
 // ---------------------------------------------------------------------------
 // PML, boundary and fault tiles
 // ---------------------------------------------------------------------------
@@ -352,13 +361,12 @@ void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
 // and R = 64/LW rows per wavefront, so narrow items (the x slabs) fill the
 // wavefront; node rows of the block are w*R + lane/LW, with halo rows 0 and W*R-1.
 template <class T, int W, int MASK>
-__global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+__device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
+                                              const Work &wk) {
     constexpr int NS = W * 64 + 64;  // LDS slots per value; lanes past R*LW use the last 64
     __shared__ T sN[7][NS];
     __shared__ T sC[21][NS];
     const int lane = threadIdx.x, w = threadIdx.y;
-    const Work wk = C.work[blockIdx.x];
-    if (wk.lz0 > wk.lz1) return;  // uniform over the block
     const int LW = wk.pad0, R = 64 / LW, NR = W * R;
     const int jj = lane % LW, rr = lane / LW;
     const bool valid = rr < R;
@@ -709,15 +717,23 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
     }
 }
 
-template <class T, int M>
-static void launch_boundary(int nwork, hipStream_t s, const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops) {
-    hipLaunchKernelGGL((k_fusedp<T, 8, M>), dim3(nwork), dim3(64, 8), 0, s, P, C, ops);
+template <class T, int W, int MASK>
+__global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+    const Work wk = C.work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;  // uniform over the block
+    boundary_item<T, W, MASK>(P, C, ops, wk);
+}
+template <class T, int W, int MASK>
+__global__ __launch_bounds__(64 * W) void k_fusedp_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work) {
+    const Work wk = work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;
+    const Scen<T> &S = sc[wk.pad1];
+    if (!S.active) return;  // uniform over the block
+    boundary_item<T, W, MASK>(S.P, S.C, S.ops, wk);
 }
 
 template <class T>
-void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3],
-                  int waves, hipStream_t s) {
-    if (nwork <= 0) return;
+FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work) {
     FusedC<T> C;
     C.work = work;
     C.gx = T(1) / (T(4) * sp[0]);
@@ -730,26 +746,71 @@ void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, 
     C.dth2 = P.hg_on ? P.dth2 : T(0);
     C.vol = sp[0] * sp[1] * sp[2];
     C.rvol = T(1) / C.vol;
-    if (mask >= 0) {  // boundary kernel (8 waves), specialised on the damped axes
-        switch (mask) {
-        case 0: launch_boundary<T, 0>(nwork, s, P, C, ops); break;
-        case 1: launch_boundary<T, 1>(nwork, s, P, C, ops); break;
-        case 2: launch_boundary<T, 2>(nwork, s, P, C, ops); break;
-        case 3: launch_boundary<T, 3>(nwork, s, P, C, ops); break;
-        case 4: launch_boundary<T, 4>(nwork, s, P, C, ops); break;
-        case 5: launch_boundary<T, 5>(nwork, s, P, C, ops); break;
-        case 6: launch_boundary<T, 6>(nwork, s, P, C, ops); break;
-        default: launch_boundary<T, 7>(nwork, s, P, C, ops); break;
-        }
-    } else {
+    return C;
+}
+
+template <class T>
+void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3],
+                  int waves, hipStream_t s) {
+    if (nwork <= 0) return;
+    const FusedC<T> C = fused_consts<T>(P, sp, work);
+    const dim3 g(nwork), b8(64, 8);
+    switch (mask) {
+    case -1:
         switch (waves) {
-        case 4: hipLaunchKernelGGL((k_fused<T, 4>), dim3(nwork), dim3(64, 4), 0, s, P, C, ops); break;
-        case 8: hipLaunchKernelGGL((k_fused<T, 8>), dim3(nwork), dim3(64, 8), 0, s, P, C, ops); break;
-        default: hipLaunchKernelGGL((k_fused<T, 16>), dim3(nwork), dim3(64, 16), 0, s, P, C, ops); break;
+        case 4: hipLaunchKernelGGL((k_fused<T, 4>), g, dim3(64, 4), 0, s, P, C, ops); break;
+        case 8: hipLaunchKernelGGL((k_fused<T, 8>), g, b8, 0, s, P, C, ops); break;
+        default: hipLaunchKernelGGL((k_fused<T, 16>), g, dim3(64, 16), 0, s, P, C, ops); break;
         }
+        break;
+    case 0: hipLaunchKernelGGL((k_fusedp<T, 8, 0>), g, b8, 0, s, P, C, ops); break;
+    case 1: hipLaunchKernelGGL((k_fusedp<T, 8, 1>), g, b8, 0, s, P, C, ops); break;
+    case 2: hipLaunchKernelGGL((k_fusedp<T, 8, 2>), g, b8, 0, s, P, C, ops); break;
+    case 3: hipLaunchKernelGGL((k_fusedp<T, 8, 3>), g, b8, 0, s, P, C, ops); break;
+    case 4: hipLaunchKernelGGL((k_fusedp<T, 8, 4>), g, b8, 0, s, P, C, ops); break;
+    case 5: hipLaunchKernelGGL((k_fusedp<T, 8, 5>), g, b8, 0, s, P, C, ops); break;
+    case 6: hipLaunchKernelGGL((k_fusedp<T, 8, 6>), g, b8, 0, s, P, C, ops); break;
+    default: hipLaunchKernelGGL((k_fusedp<T, 8, 7>), g, b8, 0, s, P, C, ops); break;
     }
 }
 
+template <class T>
+void launch_fused_batch(const Scen<T> *sc, const Work *work, int nwork, int mask, hipStream_t s) {
+    if (nwork <= 0) return;
+    const dim3 g(nwork), b8(64, 8);
+    switch (mask) {
+    case -1: hipLaunchKernelGGL((k_fused_batch<T, 8>), g, b8, 0, s, sc, work); break;
+    case 0: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 0>), g, b8, 0, s, sc, work); break;
+    case 1: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 1>), g, b8, 0, s, sc, work); break;
+    case 2: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 2>), g, b8, 0, s, sc, work); break;
+    case 3: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 3>), g, b8, 0, s, sc, work); break;
+    case 4: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 4>), g, b8, 0, s, sc, work); break;
+    case 5: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 5>), g, b8, 0, s, sc, work); break;
+    case 6: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 6>), g, b8, 0, s, sc, work); break;
+    default: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 7>), g, b8, 0, s, sc, work); break;
+    }
+}
+
+// clear the statistics slots the scenarios of a batch fill in the coming step
+// (one thread per scenario; replaces per-scenario memsets)
+template <class T>
+__global__ void k_zero_stats(const StatZero<T> *__restrict__ z, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (z[q].now) z[q].cur[0] = T(0), z[q].cur[3] = T(0);
+    if (z[q].next) z[q].nxt[1] = T(0), z[q].nxt[2] = T(0);
+}
+template <class T>
+void launch_zero_stats(const StatZero<T> *z, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL((k_zero_stats<T>), dim3((n + 63) / 64), dim3(64), 0, s, z, n);
+}
+template void launch_zero_stats<float>(const StatZero<float> *, int, hipStream_t);
+template void launch_zero_stats<double>(const StatZero<double> *, int, hipStream_t);
+
+template FusedC<float> fused_consts<float>(const Params<float> &, const float *, const Work *);
+template FusedC<double> fused_consts<double>(const Params<double> &, const double *, const Work *);
+template void launch_fused_batch<float>(const Scen<float> *, const Work *, int, int, hipStream_t);
+template void launch_fused_batch<double>(const Scen<double> *, const Work *, int, int, hipStream_t);
 template void launch_fused<float>(const Params<float> &, const PointOps<float> &, const Work *, int, int,
                                   const float *, int, hipStream_t);
 template void launch_fused<double>(const Params<double> &, const PointOps<double> &, const Work *, int, int,

@@ -230,6 +230,7 @@ struct Solver {
     hipStream_t stream = nullptr;
     static const int NSIDE = 3;            // side streams of the boundary kernels (default priority)
     hipStream_t side[NSIDE] = {};
+    bool own_streams = true;  // false when a batch lends its streams
     hipEvent_t ev_fork = nullptr, ev_join[NSIDE] = {};
     std::vector<void *> dallocs;
     Params<T> P;
@@ -257,11 +258,11 @@ struct Solver {
         if (stream) (void)hipStreamSynchronize(stream);  // the side streams join the main stream every step
         for (void *p : dallocs) (void)hipFree(p);
         for (int i = 0; i < NSIDE; i++) {
-            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (side[i] && own_streams) (void)hipStreamDestroy(side[i]);
             if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
         }
         if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_streams) (void)hipStreamDestroy(stream);
     }
 
     T *dalloc(size_t n) {
@@ -506,8 +507,13 @@ struct Solver {
         return true;
     }
 
+    // files are named relative to the run directory (the current directory
+    // unless a batch gives each scenario its own)
+    std::string dir;
+    std::string path(const std::string &f) const { return dir.empty() || f.empty() || f[0] == '/' ? f : dir + "/" + f; }
+
     void file_write(const std::string &fname, const T *buf, size_t reclen, int o, int n) {
-        FILE *fp = fopen(fname.c_str(), o > 0 ? "r+b" : "wb");
+        FILE *fp = fopen(path(fname).c_str(), o > 0 ? "r+b" : "wb");
         if (!fp) fail(SORD_EFILE, "cannot write " + fname);
         fseek(fp, (long)((size_t)o * reclen * sizeof(T)), SEEK_SET);
         size_t w = fwrite(buf, sizeof(T), reclen * (size_t)n, fp);
@@ -515,7 +521,7 @@ struct Solver {
         if (w != reclen * (size_t)n) fail(SORD_EFILE, "short write " + fname);
     }
     void file_read(const std::string &fname, T *buf, size_t reclen, int o, int n) {
-        FILE *fp = fopen(fname.c_str(), "rb");
+        FILE *fp = fopen(path(fname).c_str(), "rb");
         if (!fp) fail(SORD_EFILE, "cannot open input file " + fname);
         fseek(fp, (long)((size_t)o * reclen * sizeof(T)), SEEK_SET);
         size_t r = fread(buf, sizeof(T), reclen * (size_t)n, fp);
@@ -742,6 +748,25 @@ static int dispatch(void *h, Fn &&fn) {
     return guard([&] { fn(*H->s8); });
 }
 
+#include "sord_batch.inc"
+
+struct BatchHandle {
+    int realsize;
+    std::unique_ptr<Batch<float>> b4;
+    std::unique_ptr<Batch<double>> b8;
+};
+
+template <class Fn>
+static int dispatch_batch(void *h, Fn &&fn) {
+    if (!h) {
+        g_err = "null handle";
+        return SORD_ESTATE;
+    }
+    BatchHandle *H = (BatchHandle *)h;
+    if (H->realsize == 4) return guard([&] { fn(*H->b4); });
+    return guard([&] { fn(*H->b8); });
+}
+
 }  // namespace sord
 
 using namespace sord;
@@ -885,3 +910,114 @@ extern "C" void sord_destroy(void *h) {
     Handle *H = (Handle *)h;
     delete H;
 }
+
+// ---------------- batched ensembles ----------------
+extern "C" int sord_batch_create(int n, const char *const *texts, const char *const *dirs, int realsize, int device,
+                                 void **batch) {
+    if (!batch || !texts || n < 1) {
+        g_err = "batch: null argument or no scenario";
+        return SORD_EINPUT;
+    }
+    *batch = nullptr;
+    if (realsize != 4 && realsize != 8) {
+        g_err = "realsize must be 4 or 8";
+        return SORD_EINPUT;
+    }
+    BatchHandle *H = new BatchHandle();
+    H->realsize = realsize;
+    int rc = guard([&] {
+        int nd = 0;
+        HIPCHECK(hipGetDeviceCount(&nd));
+        if (device < 0 || device >= nd) fail(SORD_EHIP, "no such HIP device");
+        if (realsize == 4) {
+            H->b4.reset(new Batch<float>());
+            H->b4->device = device;
+        } else {
+            H->b8.reset(new Batch<double>());
+            H->b8->device = device;
+        }
+        for (int q = 0; q < n; q++) {
+            if (!texts[q]) fail(SORD_EINPUT, "batch: null sord.in text");
+            const std::string dir = dirs && dirs[q] ? dirs[q] : "";
+            if (realsize == 4) {
+                H->b4->S.emplace_back(new Solver<float>());
+                Solver<float> &s = *H->b4->S.back();
+                s.device = device, s.dir = dir;
+                s.parse(texts[q]);
+                s.setup();
+            } else {
+                H->b8->S.emplace_back(new Solver<double>());
+                Solver<double> &s = *H->b8->S.back();
+                s.device = device, s.dir = dir;
+                s.parse(texts[q]);
+                s.setup();
+            }
+        }
+    });
+    if (rc) {
+        delete H;
+        return rc;
+    }
+    *batch = H;
+    return SORD_OK;
+}
+
+extern "C" int sord_batch_set_tiles(void *batch, int index, int planes, int planes_boundary) {
+    return dispatch_batch(batch, [&](auto &B) {
+        if (index < -1 || index >= (int)B.S.size()) fail(SORD_EINPUT, "batch: no such scenario");
+        if (planes < 1 || planes > 4096 || planes_boundary < 1 || planes_boundary > 4096)
+            fail(SORD_EINPUT, "tile shape: planes 1..4096");
+        if (index >= 0) B.S[index]->tile_nz = planes, B.S[index]->tile_nz_b = planes_boundary;
+        if (B.initialized && index == -1) {  // apply the shapes set so far
+            HIPCHECK(hipSetDevice(B.device));
+            B.replan();
+        }
+    });
+}
+
+extern "C" int sord_batch_init(void *batch) {
+    return dispatch_batch(batch, [&](auto &B) { B.init(); });
+}
+
+extern "C" int sord_batch_step(void *batch, int nsteps) {
+    return dispatch_batch(batch, [&](auto &B) { B.step(nsteps); });
+}
+
+extern "C" int sord_batch_run(void *batch) {
+    return dispatch_batch(batch, [&](auto &B) {
+        B.step(1 << 30);
+        HIPCHECK(hipStreamSynchronize(B.stream));
+    });
+}
+
+extern "C" int sord_batch_bench(void *batch, int nsteps, double *ms_total, int64_t *node_updates) {
+    return dispatch_batch(batch, [&](auto &B) {
+        if (!B.initialized) fail(SORD_ESTATE, "sord_batch_bench before sord_batch_init");
+        HIPCHECK(hipSetDevice(B.device));
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        int64_t nu = 0;
+        HIPCHECK(hipStreamSynchronize(B.stream));
+        HIPCHECK(hipEventRecord(e0, B.stream));
+        for (int i = 0; i < nsteps && !B.done(); i++) {
+            for (auto &s : B.S)
+                if (s->it < s->nt) nu += (int64_t)s->nn[0] * s->nn[1] * s->nn[2];
+            B.step_one();
+        }
+        HIPCHECK(hipEventRecord(e1, B.stream));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_total = ms;
+        *node_updates = nu;
+    });
+}
+
+extern "C" int sord_batch_size(void *batch, int *n) {
+    return dispatch_batch(batch, [&](auto &B) { *n = (int)B.S.size(); });
+}
+
+extern "C" void sord_batch_destroy(void *batch) { delete (BatchHandle *)batch; }

@@ -27,4 +27,9 @@ namespace sord {
 // mask -1: interior kernel with `waves` waves per block; 0..7: boundary kernel
 // specialised on the damped axes (bit a: axis a), 8 waves per block
 template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, hipStream_t s);
+// constants of the fused kernels for one scenario; the batch launch takes a
+// device table of scenarios, each work item naming its own (Work.pad1)
+template <class T> FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work);
+template <class T> void launch_zero_stats(const StatZero<T> *z, int n, hipStream_t s);
+template <class T> void launch_fused_batch(const Scen<T> *sc, const Work *work, int nwork, int mask, hipStream_t s);
 }

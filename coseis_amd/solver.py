@@ -17,7 +17,9 @@ ERRORS = {
 }
 EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field',
-    'sord_bench', 'sord_set_fused', 'sord_set_tiles', 'sord_copy_probe', 'sord_last_error', 'sord_destroy',
+    'sord_bench', 'sord_set_fused', 'sord_set_tiles', 'sord_copy_probe',
+    'sord_batch_create', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
+    'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy', 'sord_last_error', 'sord_destroy',
 ]
 
 _lib = None
@@ -56,9 +58,20 @@ def load():
     lib.sord_last_error.restype = ctypes.c_char_p
     lib.sord_last_error.argtypes = []
     lib.sord_destroy.argtypes = [vp]
-    lib.sord_destroy.restype = None
-    for name in EXPORTS[:-2]:
+    ccp = ctypes.POINTER(ctypes.c_char_p)
+    lib.sord_batch_create.argtypes = [ctypes.c_int, ccp, ccp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.sord_batch_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.sord_batch_init.argtypes = [vp]
+    lib.sord_batch_step.argtypes = [vp, ctypes.c_int]
+    lib.sord_batch_run.argtypes = [vp]
+    lib.sord_batch_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), i64p]
+    lib.sord_batch_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    lib.sord_batch_destroy.argtypes = [vp]
+    for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int
+    lib.sord_last_error.restype = ctypes.c_char_p
+    lib.sord_destroy.restype = None
+    lib.sord_batch_destroy.restype = None
     _lib = lib
     return lib
 
@@ -130,6 +143,61 @@ class Solver(object):
     def close(self):
         if self._h:
             _lib.sord_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch(object):
+    """Many scenarios stepped together on one device (sord_batch_*): texts are
+    sord.in texts, dirs the run directory of each (its output files go
+    there).  tiles: optional list of (planes, planes_boundary) per scenario."""
+
+    def __init__(self, texts, dirs=None, realsize=4, device=0, tiles=None):
+        lib = load()
+        n = len(texts)
+        t = (ctypes.c_char_p * n)(*[x.encode() for x in texts])
+        d = (ctypes.c_char_p * n)(*[(x or '').encode() for x in (dirs or [''] * n)])
+        self._h = ctypes.c_void_p()
+        _check(lib.sord_batch_create(n, t, d, realsize, device, ctypes.byref(self._h)))
+        self.n = n
+        if tiles:
+            for q, (p, pb) in enumerate(tiles):
+                _check(lib.sord_batch_set_tiles(self._h, q, p, pb))
+        _check(lib.sord_batch_init(self._h))
+
+    def set_tiles(self, index, planes, planes_boundary):
+        """Planes per work item of one scenario; call replan() to apply."""
+        _check(_lib.sord_batch_set_tiles(self._h, index, planes, planes_boundary))
+
+    def replan(self):
+        _check(_lib.sord_batch_set_tiles(self._h, -1, 1, 1))
+
+    def step(self, n=1):
+        _check(_lib.sord_batch_step(self._h, n))
+
+    def run(self):
+        _check(_lib.sord_batch_run(self._h))
+
+    def bench(self, nsteps):
+        """Advance nsteps timed: (ms, node-updates)."""
+        ms, nu = ctypes.c_double(), ctypes.c_int64()
+        _check(_lib.sord_batch_bench(self._h, nsteps, ctypes.byref(ms), ctypes.byref(nu)))
+        return ms.value, nu.value
+
+    def close(self):
+        if self._h:
+            _lib.sord_batch_destroy(self._h)
             self._h = None
 
     def __enter__(self):

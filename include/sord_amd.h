@@ -90,6 +90,32 @@ int sord_set_tiles(void *handle, int waves, int planes, int planes_boundary);
  * counters (FETCH_SIZE, WRITE_SIZE) for the access width of the solver. */
 int sord_copy_probe(const void *src, void *dst, int64_t nbytes, int device);
 
+/* ---- batched ensembles ----
+ * No counterpart in the reference, which runs one scenario per process
+ * (cst/sord.py:558-561): n scenarios with their own sord.in texts (uneven
+ * meshes allowed) advance together on one device.  Each step runs the
+ * per-scenario phases of the time loop (i/o, sources, statistics) and one
+ * launch of the interior kernel over the interior tiles of all scenarios and
+ * one launch per boundary class, each tile naming its scenario.  Scenario q
+ * writes its output files into dirs[q] (NULL or "" : the current directory).
+ * Outputs are those of n separate sord_run calls. */
+int sord_batch_create(int n, const char *const *sord_in_texts, const char *const *dirs, int realsize, int device,
+                      void **batch);
+/* Planes per work item of scenario `index` (interior, boundary).  Before
+ * sord_batch_init it takes effect at init; afterwards, set the scenarios'
+ * shapes and then call it with index -1 to re-plan the batch.  Performance
+ * only (the autotuner's hook). */
+int sord_batch_set_tiles(void *batch, int index, int planes, int planes_boundary);
+int sord_batch_init(void *batch);
+/* Advance every scenario that has steps left by up to nsteps steps. */
+int sord_batch_step(void *batch, int nsteps);
+int sord_batch_run(void *batch);
+/* Advance nsteps steps timed with HIP events; returns the elapsed time (ms) and
+ * the node-updates done (sum over the scenarios that stepped). */
+int sord_batch_bench(void *batch, int nsteps, double *ms_total, int64_t *node_updates);
+int sord_batch_size(void *batch, int *n);
+void sord_batch_destroy(void *batch);
+
 /* Message of the last error on this thread ("" if none). */
 const char *sord_last_error(void);
 
