@@ -74,6 +74,9 @@ def parse():
     p.add_argument('--no-batch', action='store_true', help='skip the batched-ensemble comparison')
     p.add_argument('--batch-scenarios', type=int, default=64)
     p.add_argument('--batch-steps', type=int, default=100)
+    p.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient leg')
+    p.add_argument('--adjoint-n', type=int, default=200, help='adjoint mesh edge (nodes)')
+    p.add_argument('--adjoint-steps', type=int, default=1500)
     p.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                    help='only copy MIB MiB device to device (calibration of the HBM traffic counters)')
     return p.parse_args()
@@ -203,6 +206,53 @@ def batched_leg(nscen, steps, warmup=5, tune_steps=10):
     }
 
 
+def adjoint_leg(n, nt):
+    """Gradient of the waveform misfit against one plain forward run.
+
+    Workload (coseis_amd.configs.adjoint_200): n^3 nodes, dx = 50 m, nt
+    steps, free surfaces, 5% Gaussian heterogeneity (1 km correlation), 8
+    double couples, 64 surface receivers, FP32.  Observed traces come from
+    the second seeded model.  Plain: the forward run of the model, timed with
+    HIP events (sord_bench).  Gradient: the checkpointed forward (every
+    round(sqrt(nt)) steps, traces gathered each step) plus the backward sweep
+    (recomputation of each segment, adjoint steps), both HIP-event timed
+    inside the library.  Initialisation is excluded from both."""
+    import numpy as np
+    import torch
+    from coseis_amd import adjoint, configs
+    w = configs.adjoint_200(n=n, nt=nt)
+    prob = adjoint.Problem(w['prm'], w['receivers'], realsize=4)
+    obs = adjoint.traces(*w['observed'], prob)
+    s, d = prob.open(*w['model'])
+    cwd = os.getcwd()
+    try:
+        os.chdir(d)
+        ms_plain = s.bench(nt)[0]
+    finally:
+        os.chdir(cwd)
+        s.close()
+        shutil.rmtree(d)
+    m = [torch.tensor(a, requires_grad=True) for a in w['model']]
+    t0 = time.perf_counter()
+    tr = adjoint.SordGradient.apply(m[0], m[1], m[2], prob)
+    J = 0.5 * ((tr - torch.from_numpy(obs)) ** 2).sum()
+    J.backward()
+    wall = time.perf_counter() - t0
+    ms_f, ms_b = prob.last_timing
+    g = [x.grad.numpy() for x in m]
+    return round((ms_f + ms_b) / ms_plain, 3), {
+        'nodes': n ** 3, 'steps': nt, 'checkpoint_every': prob.every, 'receivers': len(w['receivers']),
+        'plain_forward_ms': round(ms_plain, 2), 'checkpointed_forward_ms': round(ms_f, 2),
+        'backward_ms': round(ms_b, 2), 'gradient_wall_s': round(wall, 3),
+        'misfit': float(J), 'grad_norm': [float(np.linalg.norm(x)) for x in g],
+        'plain_gnode_s': round(n ** 3 * nt / ms_plain / 1e6, 3),
+    }
+
+
+def progress(msg):
+    print('bench: ' + msg, file=sys.stderr, flush=True)
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -270,10 +320,16 @@ def main():
         'batched_vs_plain': None,
         'grad_over_forward': None,
     }
+    progress('plain forward: %.3f Gnode/s' % value)
     if not a.no_batch:
         line['batched_vs_plain'], line['batched'] = batched_leg(a.batch_scenarios, a.batch_steps)
+        progress('batched leg: %s x plain' % line['batched_vs_plain'])
+    if not a.no_adjoint:
+        line['grad_over_forward'], line['adjoint'] = adjoint_leg(a.adjoint_n, a.adjoint_steps)
+        progress('adjoint leg: gradient / forward %s' % line['grad_over_forward'])
     if not a.no_cpu:
         line['cpu_baseline'] = cpu_baseline(prm, files, a.cpu_steps)
+        progress('cpu baseline done')
     print(json.dumps(line))
 
 

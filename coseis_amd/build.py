@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libsord_amd.so')
-SOURCES = ['sord_kernels.hip', 'sord_fused.hip', 'sord_host.cpp']
+SOURCES = ['sord_kernels.hip', 'sord_fused.hip', 'sord_adjoint.hip', 'sord_host.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = [
     '-std=c++17', '-O3', '--offload-arch=gfx950', '-fPIC',
@@ -22,7 +22,8 @@ FLAGS = [
 
 
 def build(verbose=False):
-    objs = []
+    """Compile the sources (in parallel) and link the library."""
+    objs, procs = [], []
     for s in SOURCES:
         src = os.path.join(SRC, s)
         if not os.path.exists(src):
@@ -32,8 +33,11 @@ def build(verbose=False):
         cmd = [HIPCC] + FLAGS + ['-c', src, '-o', obj]
         if verbose:
             print(' '.join(cmd))
-        subprocess.run(cmd, check=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(obj)
+    for cmd, p in procs:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, cmd)
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '-shared', '--offload-arch=gfx950', '-fPIC', '-o', tmp] + objs
     if verbose:

@@ -131,6 +131,56 @@ def ensemble(n=64, seed=0, nt=800):
     return out
 
 
+def gaussian_field(shape, corr, rng):
+    """Zero-mean, unit-std random field with Gaussian correlation
+    exp(-r^2 / corr^2) (corr in cells): white noise filtered in the Fourier
+    domain by the square root of that correlation's spectrum."""
+    w = rng.standard_normal(shape)
+    k = [np.fft.fftfreq(n) * 2 * np.pi for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1]) * 2 * np.pi]
+    k2 = k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2
+    f = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-k2 * corr * corr / 8.0), s=shape)
+    f -= f.mean()
+    return f / f.std()
+
+
+def adjoint_200(n=200, nt=1500, seed=0, nrec=64, nsrc=8):
+    """Adjoint-gradient workload: n^3 nodes, dx = 50 m, dt = 0.0035 s, free
+    surface on every face.  Material (per cell): the homogeneous background
+    (rho 2700, vp 6000, vs 3464) times (1 + 0.05 g), g a unit-std Gaussian
+    random field with a 1 km correlation length (20 cells); the observed data
+    come from a second field of the next seed.  Sources: nsrc double couples
+    (random fault normal and slip, M0 = 1e16 N m, Brune tau = 0.1 s) at random
+    cells at least 10 cells inside; receivers: nrec random nodes of the top
+    face (l = 0) at least 10 nodes inside.  Returns a dict: prm (without
+    material), model and observed (rho, vp, vs) cell arrays, receivers (nrec,
+    3) 0-based node indices."""
+    rng = np.random.default_rng(seed)
+    cells = (n - 1,) * 3
+    bg = (2700.0, 6000.0, 3464.0)
+    models = []
+    for s in (seed, seed + 1):
+        g = 1.0 + 0.05 * gaussian_field(cells, 20.0, np.random.default_rng(1000 + s))
+        models.append(tuple(np.float32(b) * g.astype('f4') for b in bg))
+    prm = {
+        'shape': [n, n, n, nt], 'delta': [50.0, 50.0, 50.0, 0.0035],
+        'bc1': ['free'] * 3, 'bc2': ['free'] * 3, 'gam': [0.0],
+    }
+    comps = {'mxx': (0, 0), 'myy': (1, 1), 'mzz': (2, 2), 'myz': (1, 2), 'mzx': (2, 0), 'mxy': (0, 1)}
+    for f in comps:
+        prm[f] = []
+    for _ in range(nsrc):
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        slip = np.cross(nrm, rng.normal(size=3))
+        slip /= np.linalg.norm(slip)
+        x = [float(rng.integers(10, n - 11)) for _ in range(3)]
+        for f, (a, b) in comps.items():
+            prm[f].append((x + [[]], '.', float(1e16 * (slip[a] * nrm[b] + slip[b] * nrm[a])), 'brune', 0.1))
+    rec = np.zeros((nrec, 3), np.int32)
+    rec[:, :2] = rng.integers(10, n - 10, size=(nrec, 2))
+    return {'prm': prm, 'model': models[0], 'observed': models[1], 'receivers': rec}
+
+
 def write_inputs(files, realsize=4):
     for name, a in files.items():
         np.asarray(a, dtype='<f%d' % realsize).T.tofile(name)

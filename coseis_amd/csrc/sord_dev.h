@@ -75,6 +75,14 @@ struct Params {
 
 // One block of the fused kernels: node columns [jb, je] x [kb, ke] (at most
 // 62 x (W-2)) over planes [lz0, lz1]; lz0 > lz1 marks padding.
+// waves per block of the boundary kernel, and the widest column segment of its
+// work items (two packed node rows per wavefront with 4 waves)
+#ifndef SORD_BOUNDARY_WAVES
+#define SORD_BOUNDARY_WAVES 4
+#endif
+constexpr int BOUNDARY_WAVES = SORD_BOUNDARY_WAVES;
+constexpr int BOUNDARY_COLS = BOUNDARY_WAVES == 4 ? 30 : 62;
+
 struct Work {
     int jb, je, kb, ke, lz0, lz1, pad0, pad1;
 };
@@ -87,6 +95,22 @@ struct FusedC {
     T ax, ay, az;  // dx_b dx_c / 4: divergence weights
     T h1, dth2;    // hourglass(1), dt * hourglass(2)
     T vol, rvol;   // cell volume and its inverse
+};
+
+// arguments of the adjoint step kernels
+template <class T>
+struct AdjP {
+    Grid<T> g;
+    const T *uh[3], *vh[3];   // adjoint state of step it+1
+    T *uo[3], *vo[3];         // adjoint state of step it (out)
+    const T *u[3], *v[3];     // forward state S_it
+    const T *vn[3];           // forward v of S_it+1
+    const T *mr, *gam, *lam, *mu, *yy;
+    T *gl, *gmu, *gy, *gm;    // material sums: cells lambda, mu, yy; nodes 1/mass
+    T *s6[6], *h12[12];       // per cell: V * stress of r (xx yy zz yz zx xy), yy * modes of r
+    T dt, h1, dth2, vol, rd[3];  // rd: 1 / (4 d_a)
+    int lo[3], hi[3];          // real cells
+    int nlo[3], nhi[3];        // real nodes
 };
 
 // statistics slots of one scenario of a batch to clear before a step

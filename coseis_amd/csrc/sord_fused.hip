@@ -754,7 +754,7 @@ void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, 
                   int waves, hipStream_t s) {
     if (nwork <= 0) return;
     const FusedC<T> C = fused_consts<T>(P, sp, work);
-    const dim3 g(nwork), b8(64, 8);
+    const dim3 g(nwork), b8(64, 8), bb(64, BOUNDARY_WAVES);
     switch (mask) {
     case -1:
         switch (waves) {
@@ -763,31 +763,31 @@ void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, 
         default: hipLaunchKernelGGL((k_fused<T, 16>), g, dim3(64, 16), 0, s, P, C, ops); break;
         }
         break;
-    case 0: hipLaunchKernelGGL((k_fusedp<T, 8, 0>), g, b8, 0, s, P, C, ops); break;
-    case 1: hipLaunchKernelGGL((k_fusedp<T, 8, 1>), g, b8, 0, s, P, C, ops); break;
-    case 2: hipLaunchKernelGGL((k_fusedp<T, 8, 2>), g, b8, 0, s, P, C, ops); break;
-    case 3: hipLaunchKernelGGL((k_fusedp<T, 8, 3>), g, b8, 0, s, P, C, ops); break;
-    case 4: hipLaunchKernelGGL((k_fusedp<T, 8, 4>), g, b8, 0, s, P, C, ops); break;
-    case 5: hipLaunchKernelGGL((k_fusedp<T, 8, 5>), g, b8, 0, s, P, C, ops); break;
-    case 6: hipLaunchKernelGGL((k_fusedp<T, 8, 6>), g, b8, 0, s, P, C, ops); break;
-    default: hipLaunchKernelGGL((k_fusedp<T, 8, 7>), g, b8, 0, s, P, C, ops); break;
+    case 0: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 0>), g, bb, 0, s, P, C, ops); break;
+    case 1: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 1>), g, bb, 0, s, P, C, ops); break;
+    case 2: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 2>), g, bb, 0, s, P, C, ops); break;
+    case 3: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 3>), g, bb, 0, s, P, C, ops); break;
+    case 4: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 4>), g, bb, 0, s, P, C, ops); break;
+    case 5: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 5>), g, bb, 0, s, P, C, ops); break;
+    case 6: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 6>), g, bb, 0, s, P, C, ops); break;
+    default: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 7>), g, bb, 0, s, P, C, ops); break;
     }
 }
 
 template <class T>
 void launch_fused_batch(const Scen<T> *sc, const Work *work, int nwork, int mask, hipStream_t s) {
     if (nwork <= 0) return;
-    const dim3 g(nwork), b8(64, 8);
+    const dim3 g(nwork), b8(64, 8), bb(64, BOUNDARY_WAVES);
     switch (mask) {
     case -1: hipLaunchKernelGGL((k_fused_batch<T, 8>), g, b8, 0, s, sc, work); break;
-    case 0: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 0>), g, b8, 0, s, sc, work); break;
-    case 1: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 1>), g, b8, 0, s, sc, work); break;
-    case 2: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 2>), g, b8, 0, s, sc, work); break;
-    case 3: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 3>), g, b8, 0, s, sc, work); break;
-    case 4: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 4>), g, b8, 0, s, sc, work); break;
-    case 5: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 5>), g, b8, 0, s, sc, work); break;
-    case 6: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 6>), g, b8, 0, s, sc, work); break;
-    default: hipLaunchKernelGGL((k_fusedp_batch<T, 8, 7>), g, b8, 0, s, sc, work); break;
+    case 0: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 0>), g, bb, 0, s, sc, work); break;
+    case 1: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 1>), g, bb, 0, s, sc, work); break;
+    case 2: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 2>), g, bb, 0, s, sc, work); break;
+    case 3: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 3>), g, bb, 0, s, sc, work); break;
+    case 4: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 4>), g, bb, 0, s, sc, work); break;
+    case 5: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 5>), g, bb, 0, s, sc, work); break;
+    case 6: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 6>), g, bb, 0, s, sc, work); break;
+    default: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 7>), g, bb, 0, s, sc, work); break;
     }
 }
 

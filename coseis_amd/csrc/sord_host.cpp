@@ -198,6 +198,7 @@ struct TokReader {
 // ---------------------------------------------------------------------------
 template <class T>
 struct Solver {
+    using real_t = T;
     // input parameters (sord/globals.f90:5-16)
     int bc1[3], bc2[3], debug, faultopening, i1pml[3], i2pml[3], itio, itstats, mpin, mpout, n1expand[3],
         n2expand[3], npml, nproc3[3], nsource, nthread, ppml, shape[4];
@@ -213,6 +214,7 @@ struct Solver {
     int fm[3];
     // host initial state
     HArr<T> xx[3], xc[3], vc, mr, lam, mu, gam, yy, s1, s2;
+    HArr<T> rho_c;  // cell density after the limits (kept for the adjoint's chain rule)
     std::vector<T> wax[3];  // cell widths per axis (1-based cell index, size nm+1)
     std::vector<T> dn1, dn2, dc1, dc2;
     // fault (host init)
@@ -708,6 +710,7 @@ struct Solver {
 
 #include "sord_init.inc"
 #include "sord_step.inc"
+#include "sord_adjoint.inc"
 };
 
 // ---------------------------------------------------------------------------
@@ -1021,3 +1024,34 @@ extern "C" int sord_batch_size(void *batch, int *n) {
 }
 
 extern "C" void sord_batch_destroy(void *batch) { delete (BatchHandle *)batch; }
+
+// ---------------- adjoint gradient ----------------
+extern "C" int sord_adjoint_setup(void *h, int nrec, const int32_t *rec_nodes, int checkpoint_every) {
+    return dispatch(h, [&](auto &S) {
+        HIPCHECK(hipSetDevice(S.device));
+        S.adj_setup(nrec, rec_nodes, checkpoint_every);
+    });
+}
+
+extern "C" int sord_adjoint_forward(void *h, void *traces) {
+    return dispatch(h, [&](auto &S) {
+        using T = typename std::remove_reference<decltype(S)>::type::real_t;
+        HIPCHECK(hipSetDevice(S.device));
+        S.adj_forward((T *)traces);
+    });
+}
+
+extern "C" int sord_adjoint_backward(void *h, const void *residual, void *grad_rho, void *grad_vp, void *grad_vs) {
+    return dispatch(h, [&](auto &S) {
+        using T = typename std::remove_reference<decltype(S)>::type::real_t;
+        HIPCHECK(hipSetDevice(S.device));
+        S.adj_backward((const T *)residual, (T *)grad_rho, (T *)grad_vp, (T *)grad_vs);
+    });
+}
+
+extern "C" int sord_adjoint_timing(void *h, double *ms_forward, double *ms_backward) {
+    return dispatch(h, [&](auto &S) {
+        *ms_forward = S.adj.ms_fwd;
+        *ms_backward = S.adj.ms_bwd;
+    });
+}

@@ -19,7 +19,9 @@ EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field',
     'sord_bench', 'sord_set_fused', 'sord_set_tiles', 'sord_copy_probe',
     'sord_batch_create', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
-    'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy', 'sord_last_error', 'sord_destroy',
+    'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
+    'sord_adjoint_setup', 'sord_adjoint_forward', 'sord_adjoint_backward', 'sord_adjoint_timing',
+    'sord_last_error', 'sord_destroy',
 ]
 
 _lib = None
@@ -67,6 +69,10 @@ def load():
     lib.sord_batch_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), i64p]
     lib.sord_batch_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     lib.sord_batch_destroy.argtypes = [vp]
+    lib.sord_adjoint_setup.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+    lib.sord_adjoint_forward.argtypes = [vp, vp]
+    lib.sord_adjoint_backward.argtypes = [vp, vp, vp, vp, vp]
+    lib.sord_adjoint_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int
     lib.sord_last_error.restype = ctypes.c_char_p
@@ -130,6 +136,41 @@ class Solver(object):
         _check(_lib.sord_bench(self._h, nsteps, ctypes.byref(tot), ctypes.byref(ker), ctypes.byref(nodes),
                                name, 64))
         return tot.value, ker.value, nodes.value, name.value.decode()
+
+    # ---- adjoint gradient (sord_adjoint_*) ----
+    def adjoint_setup(self, receivers, every):
+        """Receivers: (n, 3) 0-based node indices; checkpoint every `every` steps."""
+        import numpy as np
+        rec = np.ascontiguousarray(receivers, dtype=np.int32)
+        self._nrec = len(rec)
+        _check(_lib.sord_adjoint_setup(self._h, self._nrec, rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                       every))
+
+    def adjoint_forward(self):
+        """Run all steps with checkpoints; returns the traces (nt, nrec, 3):
+        velocity at the receivers at the start of each step."""
+        import numpy as np
+        it, nt = self.status()
+        tr = np.zeros((nt, self._nrec, 3), dtype='f%d' % self.realsize)
+        _check(_lib.sord_adjoint_forward(self._h, tr.ctypes.data))
+        return tr
+
+    def adjoint_backward(self, residual, cells):
+        """Gradient of J given dJ/dtraces (nt, nrec, 3); returns dJ/drho,
+        dJ/dvp, dJ/dvs of shape `cells` (nx-1, ny-1, nz-1)."""
+        import numpy as np
+        res = np.ascontiguousarray(residual, dtype='f%d' % self.realsize)
+        n = cells[0] * cells[1] * cells[2]
+        g = [np.zeros(n, dtype='f%d' % self.realsize) for _ in range(3)]
+        _check(_lib.sord_adjoint_backward(self._h, res.ctypes.data, g[0].ctypes.data, g[1].ctypes.data,
+                                          g[2].ctypes.data))
+        return [x.reshape(cells[::-1]).T for x in g]
+
+    def adjoint_timing(self):
+        """Wall time (ms) of the last adjoint forward and backward."""
+        f, b = ctypes.c_double(), ctypes.c_double()
+        _check(_lib.sord_adjoint_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
 
     def field(self, name):
         """(device pointer, padded shape, strides, offset) of a device array."""

@@ -116,6 +116,27 @@ int sord_batch_bench(void *batch, int nsteps, double *ms_total, int64_t *node_up
 int sord_batch_size(void *batch, int *n);
 void sord_batch_destroy(void *batch);
 
+/* ---- adjoint gradient: no counterpart in the reference (the inversion
+ * workload's misfit gradient with respect to the material).  The handle's
+ * run must be rectilinear with uniform spacing, free-surface faces (bc 0 on
+ * all six), no fault, material from inputs only and moment-tensor sources
+ * (mxx..mxy field-I/O '.'/'+' ops); sord_init must have been called. */
+/* Receivers: nrec nodes as 0-based (j, k, l) of the global mesh; checkpoint
+ * the state every checkpoint_every steps (about sqrt(nt) is the usual
+ * choice: memory and recomputation both grow as nt/every + every). */
+int sord_adjoint_setup(void *handle, int nrec, const int32_t *rec_nodes, int checkpoint_every);
+/* Run the remaining steps, storing checkpoints; traces (host, realsize
+ * floats) receives [nt][nrec][3]: the velocity at each receiver at the
+ * start of each step. */
+int sord_adjoint_forward(void *handle, void *traces);
+/* Given residual = dJ/dtraces (host, [nt][nrec][3]), accumulate the gradient
+ * of J with respect to the cell density, P and S speeds into grad_rho,
+ * grad_vp, grad_vs (host, realsize floats, one per cell, Fortran order over
+ * the (nx-1, ny-1, nz-1) cells; overwritten). */
+int sord_adjoint_backward(void *handle, const void *residual, void *grad_rho, void *grad_vp, void *grad_vs);
+/* Wall time (ms, HIP events) of the last adjoint forward and backward. */
+int sord_adjoint_timing(void *handle, double *ms_forward, double *ms_backward);
+
 /* Message of the last error on this thread ("" if none). */
 const char *sord_last_error(void);
 
