@@ -149,8 +149,8 @@ def adjoint_200(n=200, nt=1500, seed=0, nrec=64, nsrc=8):
     (rho 2700, vp 6000, vs 3464) times (1 + 0.05 g), g a unit-std Gaussian
     random field with a 1 km correlation length (20 cells); the observed data
     come from a second field of the next seed.  Sources: nsrc double couples
-    (random fault normal and slip, M0 = 1e16 N m, Brune tau = 0.1 s) at random
-    cells at least 10 cells inside; receivers: nrec random nodes of the top
+    (random fault normal and slip, M0 = 1e16 N m, Brune tau = 0.1 s) at the
+    centres of random cells at least 10 cells inside (one cell each); receivers: nrec random nodes of the top
     face (l = 0) at least 10 nodes inside.  Returns a dict: prm (without
     material), model and observed (rho, vp, vs) cell arrays, receivers (nrec,
     3) 0-based node indices."""
@@ -173,7 +173,7 @@ def adjoint_200(n=200, nt=1500, seed=0, nrec=64, nsrc=8):
         nrm /= np.linalg.norm(nrm)
         slip = np.cross(nrm, rng.normal(size=3))
         slip /= np.linalg.norm(slip)
-        x = [float(rng.integers(10, n - 11)) for _ in range(3)]
+        x = [float(rng.integers(10, n - 11)) + 0.5 for _ in range(3)]
         for f, (a, b) in comps.items():
             prm[f].append((x + [[]], '.', float(1e16 * (slip[a] * nrm[b] + slip[b] * nrm[a])), 'brune', 0.1))
     rec = np.zeros((nrec, 3), np.int32)

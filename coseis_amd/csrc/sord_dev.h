@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #define SORD_MAXPT 64   // sub-cell / single-cell ops applied inside a kernel per step
+#define SORD_MAXBOX 8   // boxes grouping the cells of those ops
 
 template <class T>
 struct Grid {
@@ -36,8 +37,44 @@ struct PointOps {
     int op[SORD_MAXPT];      // 0 '=', 1 '+', 2 '*'
     long cell[SORD_MAXPT];   // linear index
     T val[SORD_MAXPT];
-    int lo[3], hi[3];        // bounding box (1-based local cell indices)
+    int nbox;                               // boxes holding the op cells (1-based local cell indices)
+    int blo[SORD_MAXBOX][3], bhi[SORD_MAXBOX][3];
+    int bop[SORD_MAXBOX + 1];               // the ops of box b are bop[b] .. bop[b+1]-1; all ops of
+                                            // one cell are in one box, in their order of application
 };
+
+// boxes of `cand` (bit mask) that hold cell (j, k, l)
+template <class T>
+__host__ __device__ inline unsigned ops_boxes(const PointOps<T> &ops, unsigned cand, int j, int k, int l) {
+    unsigned r = 0;
+    for (unsigned m = cand; m; m &= m - 1) {
+        const int b = __builtin_ctz(m);
+        if (j >= ops.blo[b][0] && j <= ops.bhi[b][0] && k >= ops.blo[b][1] && k <= ops.bhi[b][1] &&
+            l >= ops.blo[b][2] && l <= ops.bhi[b][2])
+            r |= 1u << b;
+    }
+    return r;
+}
+
+// boxes that meet the cell columns [j0, j1] x [k0, k1]
+template <class T>
+__host__ __device__ inline unsigned ops_boxes_jk(const PointOps<T> &ops, int j0, int j1, int k0, int k1) {
+    unsigned r = 0;
+    for (int b = 0; b < ops.nbox; b++)
+        if (j1 >= ops.blo[b][0] && j0 <= ops.bhi[b][0] && k1 >= ops.blo[b][1] && k0 <= ops.bhi[b][1]) r |= 1u << b;
+    return r;
+}
+
+// boxes of `cand` whose planes include l
+template <class T>
+__host__ __device__ inline unsigned ops_boxes_l(const PointOps<T> &ops, unsigned cand, int l) {
+    unsigned r = 0;
+    for (unsigned m = cand; m; m &= m - 1) {
+        const int b = __builtin_ctz(m);
+        if (l >= ops.blo[b][2] && l <= ops.bhi[b][2]) r |= 1u << b;
+    }
+    return r;
+}
 
 template <class T>
 struct Params {

@@ -86,8 +86,8 @@ __device__ __forceinline__ T tmax(T a, T b) {
 // source ops on one cell in the reference's units: volume-integrated strain
 // (potency), moment, stress; G is the true gradient, returns the stress
 template <class T>
-__device__ __forceinline__ void cell_ops(const PointOps<T> &ops, long o, const T G[3][3], T lm, T m, T vol,
-                                         T rvol, T s[6]) {
+__device__ __forceinline__ void cell_ops(const PointOps<T> &ops, unsigned boxes, long o, const T G[3][3], T lm,
+                                         T m, T vol, T rvol, T s[6]) {
     T w1[3], w2[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) w1[i] = G[i][i] * vol;
@@ -95,15 +95,18 @@ __device__ __forceinline__ void cell_ops(const PointOps<T> &ops, long o, const T
     w2[1] = T(0.5) * (G[0][2] * vol) + T(0.5) * (G[2][0] * vol);
     w2[2] = T(0.5) * (G[1][0] * vol) + T(0.5) * (G[0][1] * vol);
     auto apply = [&](int stage) {
-        for (int q = 0; q < ops.n; q++) {
-            if (ops.stage[q] != stage || ops.cell[q] != o) continue;
-            const int cc = ops.comp[q], opq = ops.op[q];
-            const T x = ops.val[q];
+        for (unsigned mb = boxes; mb; mb &= mb - 1) {
+            const int b = __builtin_ctz(mb);
+            for (int q = ops.bop[b]; q < ops.bop[b + 1]; q++) {
+                if (ops.stage[q] != stage || ops.cell[q] != o) continue;
+                const int cc = ops.comp[q], opq = ops.op[q];
+                const T x = ops.val[q];
 #pragma unroll
-            for (int t = 0; t < 6; t++) {
-                if (t != cc) continue;
-                T &r = t < 3 ? w1[t] : w2[t - 3];
-                r = opq == 0 ? x : opq == 1 ? r + x : r * x;
+                for (int t = 0; t < 6; t++) {
+                    if (t != cc) continue;
+                    T &r = t < 3 ? w1[t] : w2[t - 3];
+                    r = opq == 0 ? x : opq == 1 ? r + x : r * x;
+                }
             }
         }
     };
@@ -154,6 +157,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     const Grid<T> &g = P.g;
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
     const bool stat_now = P.stat_now, stat_next = P.stat_next;
+    const unsigned tile_boxes = ops.n ? ops_boxes_jk(ops, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : 0u;
     T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
 
     // node data: W = u + gamma v (strain), F = h1 u + dt h2 v (hourglass)
@@ -194,7 +198,10 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     };
     // stress and hourglass of cell (j, k, l) from the corner data of planes l
     // (Nlo) and l+1 (Nhi); E: polynomial at s_l = +1, Cc: at s_l = -1
-    auto cell_at = [&](int l, const T mat[4], const T (&Nlo)[4][6], const T (&Nhi)[4][6], T E[3][4], T Cc[3][4]) {
+    // st: the cell counts in the statistics (not the spare plane past the item's
+    // last, which may lie where another kernel's damping applies)
+    auto cell_at = [&](int l, const T mat[4], const T (&Nlo)[4][6], const T (&Nhi)[4][6], T E[3][4], T Cc[3][4],
+                       bool st) {
         const T lm = mat[0], m = mat[1], y = mat[2];
         T G[3][3];  // G[a][c]: gradient along a of component c
 #pragma unroll
@@ -207,9 +214,14 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
                               ((Nhi[2][c] - Nlo[2][c]) + (Nhi[3][c] - Nlo[3][c])));
         }
         T s[6];
-        const bool inbox = ops.n > 0 && cellv && j >= ops.lo[0] && j <= ops.hi[0] && k >= ops.lo[1] &&
-                           k <= ops.hi[1] && l >= ops.lo[2] && l <= ops.hi[2];
-        if (!inbox) {
+        // source ops: the boxes met by this tile, then those of plane l (both
+        // uniform over the block), then those holding this cell
+        unsigned bx = 0;
+        if (tile_boxes) {
+            const unsigned cl = ops_boxes_l(ops, tile_boxes, l);
+            if (cl && cellv) bx = ops_boxes(ops, cl, j, k, l);
+        }
+        if (!bx) {
             const T tr = G[0][0] + G[1][1] + G[2][2];
             const T m2 = m + m;
             s[0] = m2 * G[0][0] + lm * tr;
@@ -219,9 +231,9 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
             s[4] = m * (G[0][2] + G[2][0]);
             s[5] = m * (G[1][0] + G[0][1]);
         } else {
-            cell_ops<T>(ops, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
+            cell_ops<T>(ops, bx, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
         }
-        if (stat_now && cellv)
+        if (stat_now && cellv && st)
             st_w = tmax(st_w, (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + T(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
         const T A[3][3] = {{C.ax * s[0], C.ay * s[5], C.az * s[4]},
                            {C.ax * s[5], C.ay * s[1], C.az * s[3]},
@@ -262,7 +274,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     exchange(own6, NB);
     fetch_node(wk.lz0 + 1, uA, vA, gA);
     fetch_cell(wk.lz0, mA);
-    cell_at(wk.lz0 - 1, mB, NA, NB, E, CA);
+    cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, true);
 
     // one plane: cu..cm hold plane l+1 and cell l (arrived), nu..nm receive plane
     // l+2 and cell l+1; Nlo holds plane l, Nhi receives plane l+1; carry holds
@@ -273,7 +285,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
         fetch_node(l + 2, nu, nv, ng);
         fetch_cell(l + 1, nm);
         exchange(own6, Nhi);
-        cell_at(l, cm, Nlo, Nhi, E, Cn);
+        cell_at(l, cm, Nlo, Nhi, E, Cn, act);
         T R[3][2];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -378,6 +390,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     const bool ownjk = valid && jj >= 1 && jj <= LW - 2 && row >= 1 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
     const bool cellv = valid && jj <= LW - 2 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
     const Grid<T> &g = P.g;
+    const unsigned tile_boxes = ops.n ? ops_boxes_jk(ops, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : 0u;
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
     const int lmax = g.nm[2];
     const bool stat_now = P.stat_now, stat_next = P.stat_next;
@@ -418,9 +431,11 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     // cell and writes its PML memory: a cell on a tile edge is also computed by
     // the neighbouring tile, so every block reads the memory of the previous
     // step from gm and only the owner writes the new value to gmn (the host
-    // swaps the two after the launch).
+    // swaps the two after the launch).  st: the cell counts in the statistics
+    // (not the spare plane past the item's last, whose damping this kernel's
+    // mask may lack).
     auto cell_at = [&](int l, const T (&mat)[4], const T (&Nlo)[4][7], const T (&Nhi)[4][7], T (&E)[3][9],
-                       T (&Cc)[3][9], bool wr) {
+                       T (&Cc)[3][9], bool wr, bool st) {
         const int c3[3] = {j, k, l};
         bool real = true;
 #pragma unroll
@@ -495,9 +510,14 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
             }
         }
         T s[6];
-        const bool inbox = ops.n > 0 && cellv && j >= ops.lo[0] && j <= ops.hi[0] && k >= ops.lo[1] &&
-                           k <= ops.hi[1] && l >= ops.lo[2] && l <= ops.hi[2];
-        if (!inbox) {
+        // source ops: the boxes met by this tile, then those of plane l (both
+        // uniform over the block), then those holding this cell
+        unsigned bx = 0;
+        if (tile_boxes) {
+            const unsigned cl = ops_boxes_l(ops, tile_boxes, l);
+            if (cl && cellv) bx = ops_boxes(ops, cl, j, k, l);
+        }
+        if (!bx) {
             const T tr = G[0][0] + G[1][1] + G[2][2];
             const T m2 = m + m;
             s[0] = m2 * G[0][0] + lm * tr;
@@ -507,12 +527,12 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
             s[4] = m * (G[0][2] + G[2][0]);
             s[5] = m * (G[1][0] + G[0][1]);
         } else {
-            cell_ops<T>(ops, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
+            cell_ops<T>(ops, bx, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
         }
         if (!real)
 #pragma unroll
             for (int q = 0; q < 6; q++) s[q] = T(0);
-        if (stat_now && cellv)
+        if (stat_now && cellv && st)
             st_w = tmax(st_w, (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + T(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
         const T A[3][3] = {{C.ax * s[0], C.ay * s[5], C.az * s[4]},
                            {C.ax * s[5], C.ay * s[1], C.az * s[3]},
@@ -566,7 +586,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     exchange(fB, NB);
     fetch_node(wk.lz0 + 1, fA);
     fetch_cell(wk.lz0, mA);
-    cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, false);
+    cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, false, true);
 
     auto plane = [&](int l, bool act, const T (&cf)[7], const T (&cm)[4], T (&nf)[7], T (&nm_)[4], const T (&Nlo)[4][7],
                      T (&Nhi)[4][7], const T (&carry)[3][9], T (&Cn)[3][9]) {
@@ -575,7 +595,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
         fetch_node(l + 2, nf);
         fetch_cell(l + 1, nm_);
         exchange(own, Nhi);
-        cell_at(l, cm, Nlo, Nhi, E, Cn, ownjk && act);
+        cell_at(l, cm, Nlo, Nhi, E, Cn, ownjk && act, act);
         // plane combination, then the k stage: own part (s_k = +1) and the part
         // sent to the row above (s_k = -1): divergence x, y, z, full (c0, cj), viscous (c0, cj)
         T R[3][7];

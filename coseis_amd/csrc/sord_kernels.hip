@@ -166,19 +166,20 @@ __device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, i
     w2[2] = T(0.5) * e[0][1] + T(0.5) * e[1][0];  // xy
 
     const long o = g.at(j, k, l);
-    const bool inbox = ops.n > 0 && j >= ops.lo[0] && j <= ops.hi[0] && k >= ops.lo[1] &&
-                       k <= ops.hi[1] && l >= ops.lo[2] && l <= ops.hi[2];
+    const unsigned boxes = ops.n > 0 ? ops_boxes(ops, (1u << ops.nbox) - 1, j, k, l) : 0u;
     auto apply = [&](int stage) {
-        if (!inbox) return;
-        for (int q = 0; q < ops.n; q++) {
-            if (ops.stage[q] != stage || ops.cell[q] != o) continue;
-            const int cc = ops.comp[q], opq = ops.op[q];
-            const T x = ops.val[q];
+        for (unsigned mb = boxes; mb; mb &= mb - 1) {
+            const int b = __builtin_ctz(mb);
+            for (int q = ops.bop[b]; q < ops.bop[b + 1]; q++) {
+                if (ops.stage[q] != stage || ops.cell[q] != o) continue;
+                const int cc = ops.comp[q], opq = ops.op[q];
+                const T x = ops.val[q];
 #pragma unroll
-            for (int t = 0; t < 6; t++) {
-                if (t != cc) continue;
-                T &r = t < 3 ? w1[t] : w2[t - 3];
-                r = opq == 0 ? x : opq == 1 ? r + x : r * x;
+                for (int t = 0; t < 6; t++) {
+                    if (t != cc) continue;
+                    T &r = t < 3 ? w1[t] : w2[t - 3];
+                    r = opq == 0 ? x : opq == 1 ? r + x : r * x;
+                }
             }
         }
     };

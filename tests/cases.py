@@ -125,12 +125,38 @@ def bc_mix():
     return prm, {}
 
 
+def sources_scattered():
+    """Moment and potency sources scattered over the mesh: eleven groups of
+    op cells (single cells at cell centres, eight-cell stencils at nodes and
+    at a general point, an integer-index potency op), more groups than the
+    solver's op boxes, so the boxes merge."""
+    nx, ny, nz = 20, 18, 16
+    prm = {
+        'shape': [nx, ny, nz, 30], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['pml', 'pml', 'free'], 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.02],
+        'hourglass': [1.0, 2.0], 'itstats': 5,
+        'mxy': [([x, y, z, []], '.', 1e15 * (i + 1), 'brune', 0.03)
+                for i, (x, y, z) in enumerate([(2.5, 2.5, 2.5), (16.5, 3.5, 2.5), (3.5, 14.5, 5.5),
+                                               (15.5, 14.5, 12.5), (9.5, 8.5, 1.5), (9.5, 2.5, 13.5)])],
+        'mzz': [([5.0, 12.0, 4.0, []], '.', -2e15, 'brune', 0.03)],
+        'mxx': [([14.3, 6.6, 10.2, []], '.', 3e15, 'brune', 0.03)],
+        'myz': [([3.5, 3.5, 12.5, []], '.', 1e15, 'brune', 0.03),
+                ([16.5, 14.5, 3.5, []], '.', -1e15, 'brune', 0.03)],
+        'pxx': [([9, 9, 8, [1, 4]], '+', 1e-4)],
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin'), ([[], [], 0, []], '=>', 'vz-top.bin')],
+    }
+    return prm, {}
+
+
 CASES = {
     'explosion_pml': explosion_pml,
     'layered_free': layered_free,
     'fault_sw': fault_sw,
     'stages': stages,
     'bc_mix': bc_mix,
+    'sources_scattered': sources_scattered,
 }
 
 

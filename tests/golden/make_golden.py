@@ -7,7 +7,7 @@ directory and stores every output file it writes as one array of an .npz
 archive per case and precision: tests/golden/<case>_r4.npz, <case>_r8.npz.
 Profile files (timings) are not stored.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 import glob
 import os
@@ -52,7 +52,7 @@ def main():
         exe = os.path.join(ROOT, 'oracle', '_ref', 'r%d' % rs, 'sord.x')
         if not os.path.exists(exe):
             sys.exit('reference not built: run oracle/build_ref.sh')
-        for case in cases.CASES:
+        for case in sys.argv[1:] or cases.CASES:
             out = run_reference(case, rs, exe)
             path = os.path.join(HERE, '%s_r%d.npz' % (case, rs))
             np.savez_compressed(path, **out)
