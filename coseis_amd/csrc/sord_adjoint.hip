@@ -15,129 +15,254 @@
 // and the step's material derivatives, with G = D w and Gr = D r:
 //     dJ/dlambda_c += -V tr(G) tr(Gr),  dJ/dmu_c += -V (G + G^T) : Gr,
 //     dJ/dyy_c += -sum_q,i (P_q r_i)(P_q F_i),  dJ/dm_n += vb'_n . (v'_n - v_n) / m_n.
-// Two kernels per adjoint step: cells (gradients, stress of r, hourglass
-// modes of r, material sums) and nodes (gather, update).
+// One fused kernel per adjoint step, on the z-march tiles of the forward
+// interior kernel (sord_fused.hip): a block of W waves marches the planes l
+// of its tile; per plane every lane computes the cell whose lower corner is
+// its node from the 8 corner values of r, w and F (exchanged through LDS and
+// DPP lane shifts), and the cell's contributions to its corners are gathered
+// as polynomials in the corner signs, carried from plane to plane.  Ke r and
+// Kh r are gathered separately (they enter ub and vb with different weights).
+// Cells outside the mesh contribute nothing (free surfaces); the material
+// sums of a cell are accumulated by the block that owns its lower corner.
 #include "sord_dev.h"
 #include "sord_launch.h"
 
 namespace sord {
 
-
-// corner k (bits: dj, dk, dl) of a cell: e_a = +1 at the upper corner along a
-__device__ __forceinline__ int esgn(int kq, int a) { return (kq >> a & 1) ? 1 : -1; }
-// hourglass patterns (modes s_k s_l, s_j s_l, s_j s_k, s_j s_k s_l; s = +1 at the lower corner)
-__device__ __forceinline__ int pq(int q, int kq) {
-    const int sj = (kq & 1) ? -1 : 1, sk = (kq & 2) ? -1 : 1, sl = (kq & 4) ? -1 : 1;
-    return q == 0 ? sk * sl : q == 1 ? sj * sl : q == 2 ? sj * sk : sj * sk * sl;
+// DPP lane shifts (wave_shl:1 / wave_shr:1; the lane without a source reads 0)
+__device__ __forceinline__ int adj_dpp_next(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ int adj_dpp_prev(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ float adj_next(float x) { return __int_as_float(adj_dpp_next(__float_as_int(x))); }
+__device__ __forceinline__ float adj_prev(float x) { return __int_as_float(adj_dpp_prev(__float_as_int(x))); }
+__device__ __forceinline__ double adj_next(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = adj_dpp_next((int)b), hi = adj_dpp_next((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double adj_prev(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = adj_dpp_prev((int)b), hi = adj_dpp_prev((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// the four hourglass modes over the cell corners (planes l: f, l+1: g;
+// corner index dj + 2 dk; s = +1 at the lower corner): s_k s_l, s_j s_l,
+// s_j s_k, s_j s_k s_l
 template <class T>
-__global__ __launch_bounds__(256) void k_adj_cell(AdjP<T> A) {
-    const int j = A.lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = A.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
-    const int l = A.lo[2] + blockIdx.z;
-    if (j > A.hi[0] || k > A.hi[1]) return;
-    const Grid<T> &g = A.g;
-    T r[8][3], w[8][3], F[8][3];
-#pragma unroll
-    for (int kq = 0; kq < 8; kq++) {
-        const long o = g.at(j + (kq & 1), k + (kq >> 1 & 1), l + (kq >> 2 & 1));
-        const T m = A.mr[o], ga = A.gam[o];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const T vb = A.vh[i][o] + A.dt * A.uh[i][o];
-            r[kq][i] = A.dt * m * vb;
-            w[kq][i] = A.u[i][o] + ga * A.v[i][o];
-            F[kq][i] = A.h1 * A.u[i][o] + A.dth2 * A.v[i][o];
-        }
-    }
-    T G[3][3], Gr[3][3];  // [a][i]: derivative along a of component i
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            T x = T(0), y = T(0);
-#pragma unroll
-            for (int kq = 0; kq < 8; kq++) {
-                x += T(esgn(kq, a)) * w[kq][i];
-                y += T(esgn(kq, a)) * r[kq][i];
-            }
-            G[a][i] = A.rd[a] * x;
-            Gr[a][i] = A.rd[a] * y;
-        }
-    const long oc = g.at(j, k, l);
-    const T lm = A.lam[oc], mu = A.mu[oc], yy = A.yy[oc];
-    const T tr = G[0][0] + G[1][1] + G[2][2], trr = Gr[0][0] + Gr[1][1] + Gr[2][2];
-    T sm = T(0);
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) sm += Gr[a][i] * (G[a][i] + G[i][a]);
-    A.gl[oc] += -A.vol * tr * trr;
-    A.gmu[oc] += -A.vol * sm;
-    // V * Hooke(Gr)
-    const T V = A.vol;
-    A.s6[0][oc] = V * (T(2) * mu * Gr[0][0] + lm * trr);
-    A.s6[1][oc] = V * (T(2) * mu * Gr[1][1] + lm * trr);
-    A.s6[2][oc] = V * (T(2) * mu * Gr[2][2] + lm * trr);
-    A.s6[3][oc] = V * mu * (Gr[1][2] + Gr[2][1]);
-    A.s6[4][oc] = V * mu * (Gr[2][0] + Gr[0][2]);
-    A.s6[5][oc] = V * mu * (Gr[0][1] + Gr[1][0]);
-    T sy = T(0);
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            T hr = T(0), hf = T(0);
-#pragma unroll
-            for (int kq = 0; kq < 8; kq++) {
-                hr += T(pq(q, kq)) * r[kq][i];
-                hf += T(pq(q, kq)) * F[kq][i];
-            }
-            sy += hr * hf;
-            A.h12[3 * q + i][oc] = yy * hr;
-        }
-    A.gy[oc] += -sy;
+__device__ __forceinline__ void adj_modes(const T f[4], const T g[4], T h[4]) {
+    const T a00 = f[0] - f[1], a10 = f[2] - f[3];
+    const T a01 = g[0] - g[1], a11 = g[2] - g[3];
+    const T b00 = f[0] + f[1], b10 = f[2] + f[3];
+    const T b01 = g[0] + g[1], b11 = g[2] + g[3];
+    h[0] = (b00 - b10) - (b01 - b11);
+    h[1] = (a00 + a10) - (a01 + a11);
+    const T d0 = a00 - a10, d1 = a01 - a11;
+    h[2] = d0 + d1;
+    h[3] = d0 - d1;
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void k_adj_node(AdjP<T> A) {
-    const int j = A.nlo[0] + blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = A.nlo[1] + blockIdx.y * blockDim.y + threadIdx.y;
-    const int l = A.nlo[2] + blockIdx.z;
-    if (j > A.nhi[0] || k > A.nhi[1]) return;
+// raw node loads of one plane: ub 0-2, vb 3-5, u 6-8, v 9-11, v of the next
+// state 12-14, 1/m 15, gamma 16
+#define ADJ_NR 17
+
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__restrict__ work) {
+    const Work wk = work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;  // uniform over the block
+    __shared__ T sN[9][W][64];
+    __shared__ T sC[12][W][64];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int j = wk.jb - 1 + lane, k = wk.kb - 1 + w;
+    const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= wk.je && k <= wk.ke;
+    const bool colreal = lane <= 62 && w <= W - 2 && j <= wk.je && k <= wk.ke && j >= A.lo[0] && j <= A.hi[0] &&
+                         k >= A.lo[1] && k <= A.hi[1];
     const Grid<T> &g = A.g;
-    // stress component index of (i, a): xx yy zz yz zx xy
-    const int sidx[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
-    T q3[3] = {T(0), T(0), T(0)}, h3[3] = {T(0), T(0), T(0)};
-#pragma unroll
-    for (int kq = 0; kq < 8; kq++) {  // the node is corner kq of cell (j,k,l) - offsets
-        const int cj = j - (kq & 1), ck = k - (kq >> 1 & 1), cl = l - (kq >> 2 & 1);
-        if (cj < A.lo[0] || cj > A.hi[0] || ck < A.lo[1] || ck > A.hi[1] || cl < A.lo[2] || cl > A.hi[2]) continue;
-        const long oc = g.at(cj, ck, cl);
+    const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
+    const int lmax = g.nm[2];
+    const T dt = A.dt, h1 = A.h1, dth2 = A.dth2, V = A.vol;
+    const T ax = V * A.rd[0], ay = V * A.rd[1], az = V * A.rd[2];
+
+    auto fetch_node = [&](int l, T (&R)[ADJ_NR]) {
+        const long o = g.at(jc, kc, min(l, lmax));
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            T x = T(0);
-#pragma unroll
-            for (int a = 0; a < 3; a++) x += T(esgn(kq, a)) * A.rd[a] * A.s6[sidx[i][a]][oc];
-            q3[i] -= x;
-            T y = T(0);
-#pragma unroll
-            for (int q = 0; q < 4; q++) y += T(pq(q, kq)) * A.h12[3 * q + i][oc];
-            h3[i] -= y;
+            R[i] = A.uh[i][o], R[3 + i] = A.vh[i][o], R[6 + i] = A.u[i][o], R[9 + i] = A.v[i][o];
+            R[12 + i] = A.vn[i][o];
         }
-    }
-    const long o = g.at(j, k, l);
-    const T m = A.mr[o], ga = A.gam[o];
-    T gm = T(0);
+        R[15] = A.mr[o], R[16] = A.gam[o];
+    };
+    auto fetch_cell = [&](int l, T (&M)[3]) {
+        const long o = g.at(jc, kc, min(l, lmax));
+        M[0] = A.lam[o], M[1] = A.mu[o], M[2] = A.yy[o];
+    };
+    // exchanged: r = dt m (vb + dt ub) 0-2, w = u + gamma v 3-5, F = h1 u + dth2 v 6-8
+    auto node_vals = [&](const T (&R)[ADJ_NR], T (&o9)[9]) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const T vb = A.vh[i][o] + A.dt * A.uh[i][o];
-        A.uo[i][o] = A.uh[i][o] + q3[i] + A.h1 * h3[i];
-        A.vo[i][o] = vb + ga * q3[i] + A.dth2 * h3[i];
-        gm += vb * (A.vn[i][o] - A.v[i][o]);
+        for (int i = 0; i < 3; i++) {
+            const T vbp = R[3 + i] + dt * R[i];
+            o9[i] = dt * R[15] * vbp;
+            o9[3 + i] = R[6 + i] + R[16] * R[9 + i];
+            o9[6 + i] = h1 * R[6 + i] + dth2 * R[9 + i];
+        }
+    };
+    auto exchange = [&](const T (&o9)[9], T (&N)[4][9]) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) sN[q][w][lane] = o9[q];
+        __syncthreads();
+        const int wa = w + 1 < W ? w + 1 : w;
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const T ab = sN[q][wa][lane];
+            N[0][q] = o9[q];
+            N[2][q] = ab;
+            N[1][q] = adj_next(o9[q]);
+            N[3][q] = adj_next(ab);
+        }
+    };
+    // cell (j, k, l): polynomials of Ke r (E: 1, s_j, s_k at s_l = +1; Ce at
+    // s_l = -1) and of Kh r (H: s_j, s_k, s_j s_k; Ch), material sums if acc
+    auto cell_at = [&](int l, const T (&M)[3], const T (&Nlo)[4][9], const T (&Nhi)[4][9], T (&E)[3][3],
+                       T (&Ce)[3][3], T (&H)[3][3], T (&Ch)[3][3], bool acc) {
+        const bool real = colreal && l >= A.lo[2] && l <= A.hi[2];
+        T Gr[3][3], G[3][3];  // [a][c]: derivative along a of component c
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                const int q = f ? 3 + c : c;
+                const T gxv = A.rd[0] * (((Nlo[1][q] - Nlo[0][q]) + (Nlo[3][q] - Nlo[2][q])) +
+                                         ((Nhi[1][q] - Nhi[0][q]) + (Nhi[3][q] - Nhi[2][q])));
+                const T gyv = A.rd[1] * (((Nlo[2][q] - Nlo[0][q]) + (Nlo[3][q] - Nlo[1][q])) +
+                                         ((Nhi[2][q] - Nhi[0][q]) + (Nhi[3][q] - Nhi[1][q])));
+                const T gzv = A.rd[2] * (((Nhi[0][q] - Nlo[0][q]) + (Nhi[1][q] - Nlo[1][q])) +
+                                         ((Nhi[2][q] - Nlo[2][q]) + (Nhi[3][q] - Nlo[3][q])));
+                if (f) G[0][c] = gxv, G[1][c] = gyv, G[2][c] = gzv;
+                else Gr[0][c] = gxv, Gr[1][c] = gyv, Gr[2][c] = gzv;
+            }
+        }
+        const T lm = real ? M[0] : T(0), m = real ? M[1] : T(0), y = real ? M[2] : T(0);
+        const T trr = Gr[0][0] + Gr[1][1] + Gr[2][2];
+        const T m2 = m + m;
+        T s[6];
+        s[0] = m2 * Gr[0][0] + lm * trr;
+        s[1] = m2 * Gr[1][1] + lm * trr;
+        s[2] = m2 * Gr[2][2] + lm * trr;
+        s[3] = m * (Gr[2][1] + Gr[1][2]);
+        s[4] = m * (Gr[0][2] + Gr[2][0]);
+        s[5] = m * (Gr[1][0] + Gr[0][1]);
+        const T Av[3][3] = {{ax * s[0], ay * s[5], az * s[4]},
+                            {ax * s[5], ay * s[1], az * s[3]},
+                            {ax * s[4], ay * s[3], az * s[2]}};
+        T sy = T(0);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            T fl[4], fh[4], hr[4], hf[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) fl[q] = Nlo[q][c], fh[q] = Nhi[q][c];
+            adj_modes<T>(fl, fh, hr);
+#pragma unroll
+            for (int q = 0; q < 4; q++) fl[q] = Nlo[q][6 + c], fh[q] = Nhi[q][6 + c];
+            adj_modes<T>(fl, fh, hf);
+#pragma unroll
+            for (int q = 0; q < 4; q++) sy = sy + hr[q] * hf[q];
+            T h[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) h[q] = y * hr[q];
+            E[c][0] = Av[c][2], E[c][1] = Av[c][0], E[c][2] = Av[c][1];
+            Ce[c][0] = -Av[c][2], Ce[c][1] = Av[c][0], Ce[c][2] = Av[c][1];
+            H[c][0] = -h[1], H[c][1] = -h[0], H[c][2] = -h[2] - h[3];
+            Ch[c][0] = h[1], Ch[c][1] = h[0], Ch[c][2] = -h[2] + h[3];
+        }
+        if (acc && real) {
+            const long o = g.at(j, k, l);
+            const T tr = G[0][0] + G[1][1] + G[2][2];
+            T sm = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) sm = sm + Gr[a][c] * (G[a][c] + G[c][a]);
+            A.gl[o] += -V * tr * trr;
+            A.gmu[o] += -V * sm;
+            A.gy[o] += -sy;
+        }
+    };
+
+    T o9[9];
+    T NA[4][9], NB[4][9];
+    T EA[3][3], HA[3][3];                              // polynomials of the current cell (s_l = +1)
+    T CeA[3][3], ChA[3][3], CeB[3][3], ChB[3][3];      // s_l = -1 polynomials (carried, new in turn)
+    T RA[ADJ_NR], RB[ADJ_NR], mA[3], mB[3];
+    T ow[14];  // own node of the plane being updated: ub, vb, v, vn, 1/m, gamma
+    auto keep_own = [&](const T (&R)[ADJ_NR]) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) ow[i] = R[i];
+#pragma unroll
+        for (int i = 0; i < 3; i++) ow[6 + i] = R[9 + i], ow[9 + i] = R[12 + i];
+        ow[12] = R[15], ow[13] = R[16];
+    };
+    fetch_node(wk.lz0 - 1, RA);
+    node_vals(RA, o9);
+    exchange(o9, NA);
+    fetch_node(wk.lz0, RB);
+    fetch_cell(wk.lz0 - 1, mB);
+    node_vals(RB, o9);
+    keep_own(RB);
+    __syncthreads();  // every wave has read the first plane from LDS
+    exchange(o9, NB);
+    fetch_node(wk.lz0 + 1, RA);
+    fetch_cell(wk.lz0, mA);
+    cell_at(wk.lz0 - 1, mB, NA, NB, EA, CeA, HA, ChA, false);
+
+    auto plane = [&](int l, bool act, const T (&Rc)[ADJ_NR], const T (&mc)[3], T (&Rn)[ADJ_NR], T (&mn)[3],
+                     const T (&Nlo)[4][9], T (&Nhi)[4][9], const T (&cE)[3][3], const T (&cH)[3][3], T (&nE)[3][3],
+                     T (&nH)[3][3]) {
+        node_vals(Rc, o9);
+        fetch_node(l + 2, Rn);
+        fetch_cell(l + 1, mn);
+        exchange(o9, Nhi);
+        cell_at(l, mc, Nlo, Nhi, EA, nE, HA, nH, own && act);
+        T Re[3][2], Rh[3][2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const T e0 = EA[c][0] + cE[c][0], ej = EA[c][1] + cE[c][1], ek = EA[c][2] + cE[c][2];
+            const T hj = HA[c][0] + cH[c][0], hk = HA[c][1] + cH[c][1], hjk = HA[c][2] + cH[c][2];
+            Re[c][0] = e0 + ek;
+            Re[c][1] = ej;
+            Rh[c][0] = hk;
+            Rh[c][1] = hj + hjk;
+            sC[4 * c][w][lane] = e0 - ek;
+            sC[4 * c + 1][w][lane] = ej;
+            sC[4 * c + 2][w][lane] = -hk;
+            sC[4 * c + 3][w][lane] = hj - hjk;
+        }
+        __syncthreads();
+        const int wb = w > 0 ? w - 1 : 0;
+        T fe[3], fh[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const T r0 = Re[c][0] + sC[4 * c][wb][lane], rj = Re[c][1] + sC[4 * c + 1][wb][lane];
+            fe[c] = (r0 + rj) + adj_prev(r0 - rj);
+            const T q0 = Rh[c][0] + sC[4 * c + 2][wb][lane], qj = Rh[c][1] + sC[4 * c + 3][wb][lane];
+            fh[c] = (q0 + qj) + adj_prev(q0 - qj);
+        }
+        if (own && act) {
+            const long o = g.at(j, k, l);
+            T gm = T(0);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const T vbp = ow[3 + c] + dt * ow[c];
+                A.uo[c][o] = ow[c] + fe[c] + h1 * fh[c];
+                A.vo[c][o] = vbp + ow[13] * fe[c] + dth2 * fh[c];
+                gm = gm + vbp * (ow[9 + c] - ow[6 + c]);
+            }
+            if (ow[12] != T(0)) A.gm[o] += gm / ow[12];
+        }
+        keep_own(Rc);
+    };
+    for (int l = wk.lz0; l <= wk.lz1; l += 2) {
+        plane(l, true, RA, mA, RB, mB, NB, NA, CeA, ChA, CeB, ChB);
+        plane(l + 1, l + 1 <= wk.lz1, RB, mB, RA, mA, NA, NB, CeB, ChB, CeA, ChA);
     }
-    if (m != T(0)) A.gm[o] += gm / m;
 }
 
 // add w * r to the adjoint velocity at receiver nodes
@@ -161,12 +286,8 @@ __global__ void k_rec_gather(const T *v0, const T *v1, const T *v2, Grid<T> g, c
 }
 
 template <class T>
-void launch_adjoint(const AdjP<T> &A, hipStream_t s) {
-    const dim3 b(64, 4);
-    const dim3 gc((A.hi[0] - A.lo[0] + 64) / 64, (A.hi[1] - A.lo[1] + 4) / 4, A.hi[2] - A.lo[2] + 1);
-    const dim3 gn((A.nhi[0] - A.nlo[0] + 64) / 64, (A.nhi[1] - A.nlo[1] + 4) / 4, A.nhi[2] - A.nlo[2] + 1);
-    hipLaunchKernelGGL((k_adj_cell<T>), gc, b, 0, s, A);
-    hipLaunchKernelGGL((k_adj_node<T>), gn, b, 0, s, A);
+void launch_adjoint(const AdjP<T> &A, const Work *work, int nwork, hipStream_t s) {
+    if (nwork > 0) hipLaunchKernelGGL((k_adj_fused<T, 8>), dim3(nwork), dim3(64, 8), 0, s, A, work);
 }
 template <class T>
 void launch_adj_inject(T *const vh[3], const Grid<T> &g, const int *rec, int n, const T *res, hipStream_t s) {
@@ -178,7 +299,7 @@ void launch_rec_gather(T *const v[3], const Grid<T> &g, const int *rec, int n, T
 }
 
 #define INSTA(T)                                                                                       \
-    template void launch_adjoint<T>(const AdjP<T> &, hipStream_t);                                     \
+    template void launch_adjoint<T>(const AdjP<T> &, const Work *, int, hipStream_t);                 \
     template void launch_adj_inject<T>(T *const *, const Grid<T> &, const int *, int, const T *, hipStream_t); \
     template void launch_rec_gather<T>(T *const *, const Grid<T> &, const int *, int, T *, hipStream_t);
 INSTA(float)

@@ -144,7 +144,6 @@ struct AdjP {
     const T *vn[3];           // forward v of S_it+1
     const T *mr, *gam, *lam, *mu, *yy;
     T *gl, *gmu, *gy, *gm;    // material sums: cells lambda, mu, yy; nodes 1/mass
-    T *s6[6], *h12[12];       // per cell: V * stress of r (xx yy zz yz zx xy), yy * modes of r
     T dt, h1, dth2, vol, rd[3];  // rd: 1 / (4 d_a)
     int lo[3], hi[3];          // real cells
     int nlo[3], nhi[3];        // real nodes

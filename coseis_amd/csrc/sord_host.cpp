@@ -249,6 +249,7 @@ struct Solver {
     T *red_val = nullptr;
     long *red_loc = nullptr;
     bool use_fused = true;
+    bool use_side = true;  // boundary launches on the side streams (else in order on the main stream)
     bool initialized = false;
     int state_bc_local = 1;  // all boundary conditions plane-local (-1, 1, 3, 0, pml)
 
@@ -872,6 +873,10 @@ extern "C" int sord_set_fused(void *h, int on) {
         S.use_fused = on != 0;
         if (S.initialized) S.plan_regions();
     });
+}
+
+extern "C" int sord_set_streams(void *h, int side) {
+    return dispatch(h, [side](auto &S) { S.use_side = side != 0; });
 }
 
 extern "C" int sord_set_tiles(void *h, int waves, int planes, int planes_boundary) {

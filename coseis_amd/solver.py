@@ -17,7 +17,7 @@ ERRORS = {
 }
 EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field',
-    'sord_bench', 'sord_set_fused', 'sord_set_tiles', 'sord_copy_probe',
+    'sord_bench', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_copy_probe',
     'sord_batch_create', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
     'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
     'sord_adjoint_setup', 'sord_adjoint_forward', 'sord_adjoint_backward', 'sord_adjoint_timing',
@@ -56,6 +56,7 @@ def load():
                                ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_char_p, ctypes.c_int]
     lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
     lib.sord_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.sord_set_streams.argtypes = [vp, ctypes.c_int]
     lib.sord_copy_probe.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     lib.sord_last_error.restype = ctypes.c_char_p
     lib.sord_last_error.argtypes = []
@@ -101,13 +102,15 @@ class Solver(object):
     output files, as the reference ``sord.x`` writes them into its cwd.
     """
 
-    def __init__(self, text, realsize=4, device=0, fused=True, tiles=None):
+    def __init__(self, text, realsize=4, device=0, fused=True, tiles=None, side_streams=True):
         lib = load()
         self._h = ctypes.c_void_p()
         _check(lib.sord_create(text.encode(), realsize, device, ctypes.byref(self._h)))
         self.realsize = realsize
         if not fused:
             _check(lib.sord_set_fused(self._h, 0))
+        if not side_streams:
+            _check(lib.sord_set_streams(self._h, 0))
         if tiles:
             self.set_tiles(*tiles)
         _check(lib.sord_init(self._h))

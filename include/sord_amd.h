@@ -76,6 +76,10 @@ int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, in
 /* Select the execution path: 1 fused interior kernel where it applies
  * (default), 0 stage kernels everywhere. */
 int sord_set_fused(void *handle, int on);
+/* side = 0: the boundary kernels run in order on the handle's stream before
+ * the interior kernel, instead of on three side streams beside it (slower;
+ * no cross-stream ordering).  Results are the same; default 1. */
+int sord_set_streams(void *handle, int side);
 
 /* Tile shape of the fused kernels: waves per block of the interior kernel
  * (4, 8 or 16; a block advances waves-2 node rows) and the planes one block

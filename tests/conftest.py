@@ -44,7 +44,7 @@ def collect(d, files, realsize):
     return out
 
 
-def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None):
+def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, side_streams=True):
     """Stage a case in a scratch directory and run it; returns {name: array}."""
     prm, files = cases.CASES[case]()
     if prm_update:
@@ -60,7 +60,7 @@ def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None):
             subprocess.run([oracle_exe(realsize)], check=True, stdout=subprocess.DEVNULL, env=env)
         elif runner == 'hip':
             from coseis_amd import solver
-            with solver.Solver(text, realsize=realsize, fused=fused, tiles=tiles) as s:
+            with solver.Solver(text, realsize=realsize, fused=fused, tiles=tiles, side_streams=side_streams) as s:
                 s.run()
         else:
             raise ValueError(runner)

@@ -82,3 +82,14 @@ def test_results_do_not_depend_on_tiles(case):
         out = run_case(case, 'hip', 8, tiles=tiles)
         for k in base:
             assert rel_l2(out[k], base[k]) < bar, (tiles, k, rel_l2(out[k], base[k]))
+
+
+@pytest.mark.parametrize('case', ['explosion_pml', 'layered_free'])
+def test_side_streams_do_not_change_results(case):
+    """The boundary kernels on side streams beside the interior kernel, or in
+    order on one stream: the same kernels compute every node, so the outputs
+    are identical (FP64)."""
+    a = run_case(case, 'hip', 8)
+    b = run_case(case, 'hip', 8, side_streams=False)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), (k, rel_l2(a[k], b[k]))
