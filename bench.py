@@ -23,6 +23,12 @@ same 4-byte access width), read from the committed summary in profiles/.
 step_roofline: the whole step, 68 B per node plus 48 B per node and damped
 axis (PML memory g and p), over the measured step time.
 
+batched_vs_plain: the batched ensemble (64 seeded scenarios, 48..160 nodes
+per edge, PML, FP32; batched_leg) against the same scenarios run one after
+another.  grad_over_forward: the adjoint gradient on 200^3 nodes (free
+surfaces, 8 double couples, 64 surface receivers, sqrt(N) checkpoints;
+adjoint_leg) against one plain forward run of the same model.
+
 cpu_baseline: the reference solver itself (the amdflang -O3 -fopenmp build of
 the reference Fortran, oracle/_ref/r4/sord.x) on this machine's host cores, on
 a bounded sample of the same workload (the same mesh and sources, a few time
@@ -244,7 +250,7 @@ def adjoint_leg(n, nt):
         'nodes': n ** 3, 'steps': nt, 'checkpoint_every': prob.every, 'receivers': len(w['receivers']),
         'plain_forward_ms': round(ms_plain, 2), 'checkpointed_forward_ms': round(ms_f, 2),
         'backward_ms': round(ms_b, 2), 'gradient_wall_s': round(wall, 3),
-        'misfit': float(J), 'grad_norm': [float(np.linalg.norm(x)) for x in g],
+        'misfit': float(J.detach()), 'grad_norm': [float(np.linalg.norm(x)) for x in g],
         'plain_gnode_s': round(n ** 3 * nt / ms_plain / 1e6, 3),
     }
 
