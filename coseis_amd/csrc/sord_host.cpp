@@ -886,6 +886,7 @@ extern "C" int sord_set_tiles(void *h, int waves, int planes, int planes_boundar
             fail(SORD_EINPUT, "tile shape: waves 4, 8 or 16, planes 1..4096");
         S.tile_waves = waves;
         S.tile_nz = planes, S.tile_nz_b = planes_boundary;
+        S.tiles_user = true;
         if (S.initialized) {
             HIPCHECK(hipSetDevice(S.device));
             HIPCHECK(hipStreamSynchronize(S.stream));
