@@ -1,12 +1,14 @@
 """
 Benchmark of the SORD solver step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload layered|explosion|rupture]
 
 Workload: the layered half-space model, 321 x 321 x 161 nodes, dx = 50 m,
 dt = 0.004 s, free surface on top, 10-node PML elsewhere, viscous hourglass,
 strike-slip double couple at 2 km, 16 x 16 surface receivers, FP32
-(coseis_amd.configs.layered_321).  The run is staged with W + K time steps;
+(coseis_amd.configs.layered_321; --workload explosion / rupture time the
+61^3 explosion and the 400x200x200 rupture configurations instead, with the
+same contract).  The run is staged with W + K time steps;
 W steps run untimed, then exactly K steps are timed (HIP events on the solver
 stream, bracketed by device synchronisation), with their field i/o and
 statistics as the reference performs them.  value = node-updates per second
@@ -20,6 +22,9 @@ events on its stream; the boundary kernels run beside it on side streams, so
 the duration includes that sharing).  traffic: HBM bytes per launch from
 rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy with the
 same 4-byte access width), read from the committed summary in profiles/.
+roofline.alone: the same kernel over ALONE_STEPS further steps run with every
+kernel in order on one stream (sord_set_streams), so that its duration is its
+own, not shared with the boundary kernels.
 step_roofline: the whole step, 68 B per node plus 48 B per node and damped
 axis (PML memory g and p), over the measured step time.
 
@@ -49,6 +54,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 ALG_BYTES = {4: 68, 8: 136}
+ALONE_STEPS = 50  # untimed for `value`: the dominant kernel measured on one stream after the timed steps
+WORKLOADS = {
+    'layered': ('layered_321', 'layered half-space 321x321x161, dx 50 m, dt 0.004 s, free surface + '
+                               '10-node PML, viscous hourglass, double couple, 16x16 receivers'),
+    'explosion': ('explosion_61', 'explosion in a full space 61^3, dx 100 m, dt 0.0075 s, 10-node PML, '
+                                  'isotropic moment 1e16 N m'),
+    'rupture': ('rupture_400', 'spontaneous rupture 400x200x200, dx 100 m, dt 0.008 s, 10-node PML, '
+                               'slip-weakening vertical fault 30x15 km with nucleation'),
+}
 PML_BYTES = {4: 48, 8: 96}  # per node and damped axis: read + write of g and p (3 components each)
 
 
@@ -72,6 +86,8 @@ def parse():
     p.add_argument('--steps', type=int, default=1500)
     p.add_argument('--warmup', type=int, default=20)
     p.add_argument('--realsize', type=int, default=4)
+    p.add_argument('--workload', default='layered', choices=sorted(WORKLOADS),
+                   help='plain-forward workload (layered: the metric; explosion, rupture: the other configurations)')
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--cpu-steps', type=int, default=6)
     p.add_argument('--stage-only', action='store_true', help='stage kernels only (no fused kernel)')
@@ -270,8 +286,9 @@ def main():
         sys.exit(2)
     import torch
     from coseis_amd import configs, solver, sord
-    nt = a.warmup + a.steps
-    prm, files = configs.layered_321(nt=nt)
+    alone_steps = 0 if a.stage_only else ALONE_STEPS
+    nt = a.warmup + a.steps + alone_steps
+    prm, files = getattr(configs, WORKLOADS[a.workload][0])(nt=nt)
     nx, ny, nz = prm['shape'][:3]
     d = tempfile.mkdtemp(prefix='sord-bench-')
     cwd = os.getcwd()
@@ -288,6 +305,10 @@ def main():
         ms_total, ms_kernel, knodes, kname = s.bench(a.steps)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        ms_alone = None
+        if alone_steps:  # the dominant kernel with the GPU to itself (after the timed steps)
+            s.set_streams(False)
+            ms_alone = s.bench(alone_steps)[1]
         s.close()
     finally:
         os.chdir(cwd)
@@ -304,8 +325,7 @@ def main():
         'ms_per_step': round(ms_total / a.steps, 4),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32' if a.realsize == 4 else 'f64', 'data': 'synthetic',
-        'config': {'workload': 'layered half-space 321x321x161, dx 50 m, dt 0.004 s, free surface + '
-                               '10-node PML, viscous hourglass, double couple, 16x16 receivers',
+        'config': {'workload': WORKLOADS[a.workload][1],
                    'nodes': nodes, 'realsize': a.realsize,
                    'path': 'stage kernels' if a.stage_only else
                            'fused interior kernel + fused boundary kernels on side streams'},
@@ -317,6 +337,9 @@ def main():
             'traffic': traffic_from_profile(),
             'alg_bytes_per_launch': alg, 'nodes_per_launch': knodes,
             'launch_ms': round(ms_kernel, 5),
+            'alone': None if not ms_alone else {
+                'launch_ms': round(ms_alone, 5), 'achieved': round(alg / (ms_alone / 1e3) / 1e9, 1),
+                'frac': round(alg / (ms_alone / 1e3) / 1e9 / HBM_PEAK_GBS, 4), 'steps': alone_steps},
         },
         'step_roofline': {
             'alg_bytes_per_step': step_bytes,

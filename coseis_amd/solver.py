@@ -115,6 +115,10 @@ class Solver(object):
             self.set_tiles(*tiles)
         _check(lib.sord_init(self._h))
 
+    def set_streams(self, side=True):
+        """Boundary kernels on side streams (default) or in order on one stream."""
+        _check(_lib.sord_set_streams(self._h, 1 if side else 0))
+
     def set_tiles(self, waves=8, planes=16, planes_boundary=16):
         """Tile shape of the fused kernels (performance only; see sord_set_tiles)."""
         _check(_lib.sord_set_tiles(self._h, waves, planes, planes_boundary))
