@@ -138,7 +138,7 @@ def gaussian_field(shape, corr, rng):
     w = rng.standard_normal(shape)
     k = [np.fft.fftfreq(n) * 2 * np.pi for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1]) * 2 * np.pi]
     k2 = k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2
-    f = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-k2 * corr * corr / 8.0), s=shape)
+    f = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-k2 * corr * corr / 8.0), s=shape, axes=(0, 1, 2))
     f -= f.mean()
     return f / f.std()
 
