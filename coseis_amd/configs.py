@@ -182,5 +182,7 @@ def adjoint_200(n=200, nt=1500, seed=0, nrec=64, nsrc=8):
 
 
 def write_inputs(files, realsize=4):
+    """Input files: reals in the run's precision, integer arrays as int32."""
     for name, a in files.items():
-        np.asarray(a, dtype='<f%d' % realsize).T.tofile(name)
+        a = np.asarray(a)
+        a.astype('<i4' if np.issubdtype(a.dtype, np.integer) else '<f%d' % realsize).T.tofile(name)

@@ -76,9 +76,26 @@ __host__ __device__ inline unsigned ops_boxes_l(const PointOps<T> &ops, unsigned
     return r;
 }
 
+// Finite kinematic source (sord/kinematic_source.f90): n subfaults; xi are
+// local cell (potency, moment) or node (force) coordinates; w[6] the tensor
+// (xx yy zz yz zx xy) or force (x y z) weights; history samples of subfault q
+// from off[q].
+template <class T>
+struct FSrc {
+    int n, kind;  // kind 1 potency, 2 moment, 3 force
+    const T *xi[3], *w[6], *t0, *dt, *hist;
+    const int *nt, *off;
+    int lo[3], hi[3];  // cell or node range of the mesh
+};
+
 template <class T>
 struct Params {
     Grid<T> g;
+    // finite source values of the current step over the box fs_lo + [0, fs_n)
+    // (cells), added to the potency (fs_kind 1) or moment (2) by the stress
+    // kernel; a force source (3) goes straight into the force
+    int fs_kind, fs_lo[3], fs_n[3];
+    T *fs[6];
     T dt;
     // node vectors (current state) and next-state buffers
     T *u[3], *v[3], *un[3], *vn[3];

@@ -237,6 +237,11 @@ struct Solver {
     hipEvent_t ev_fork = nullptr, ev_join[NSIDE] = {};
     std::vector<void *> dallocs;
     Params<T> P;
+    FSrc<T> fsrc = {};                  // finite source on the device (fsrc.n = 0: none)
+    struct {
+        std::vector<T> xi[3], w[6], t0, dt, hist;
+        std::vector<int> nt, off;
+    } fsh;                              // its host arrays until the upload
     Fault<T> F;
     View vview;          // volume arrays
     View fview;          // fault planes
@@ -422,7 +427,6 @@ struct Solver {
             if (!(t2 == 0 || t2 == 1 || t2 == -1 || t2 == 3)) state_bc_local = 0;
         }
         if (itio < 1 || itstats < 1) fail(SORD_EINPUT, "itio and itstats must be positive");
-        if (nsource != 0) fail(SORD_EUNSUPPORTED, "finite kinematic source (nsource != 0)");
         if (gridnoise > 0) fail(SORD_EUNSUPPORTED, "gridnoise (values of the Fortran random number generator)");
         for (int a = 0; a < 3; a++) fm[a] = nm[a];
         if (ifn) fm[ifn - 1] = 1;

@@ -183,6 +183,17 @@ __device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, i
             }
         }
     };
+    // finite source values of this step (sord/kinematic_source.f90:67-118),
+    // added before the field-I/O ops of their stage, as the reference calls it
+    auto add_fs = [&](int kind) {
+        if (P.fs_kind != kind) return;
+        const int dj = j - P.fs_lo[0], dk = k - P.fs_lo[1], dl = l - P.fs_lo[2];
+        if (dj < 0 || dj >= P.fs_n[0] || dk < 0 || dk >= P.fs_n[1] || dl < 0 || dl >= P.fs_n[2]) return;
+        const long q = dj + (long)P.fs_n[0] * (dk + (long)P.fs_n[1] * dl);
+#pragma unroll
+        for (int i = 0; i < 3; i++) w1[i] = w1[i] + P.fs[i][q], w2[i] = w2[i] + P.fs[3 + i][q];
+    };
+    add_fs(1);
     apply(0);
     if (MODE == 1) {
 #pragma unroll
@@ -200,6 +211,7 @@ __device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, i
         w1[i] = m2 * w1[i] + tr;
         w2[i] = m2 * w2[i];
     }
+    add_fs(2);
     apply(1);
     if (MODE == 2) {
 #pragma unroll
@@ -859,6 +871,54 @@ static inline dim3 box_grid(int3 lo, int3 hi, dim3 blk) {
     return dim3((hi.x - lo.x + blk.x) / blk.x, (hi.y - lo.y + blk.y) / blk.y, hi.z - lo.z + 1);
 }
 
+// Finite kinematic source (sord/kinematic_source.f90:67-118), one thread per
+// subfault: the history value at tm (linear between samples), spread over the
+// 2 x 2 x 2 cells or nodes around the subfault with trilinear weights, added
+// to the step's source box (potency, moment: minus the tensor) or to the
+// force.  Subfaults that share a cell add atomically.
+template <class T>
+__global__ __launch_bounds__(256) void k_finite_source(FSrc<T> S, T tm, Params<T> P) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= S.n) return;
+    const T t0 = S.t0[q], dt = S.dt[q];
+    int i = (int)floor((tm - t0) / dt) + 1;
+    T xi[3];
+    int i1[3], i2[3];
+    bool ok = i >= 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        xi[a] = S.xi[a][q];
+        const int fl = (int)floor(xi[a]);
+        i1[a] = max(S.lo[a], fl);
+        i2[a] = min(S.hi[a], fl + 1);
+        if (i2[a] < i1[a]) ok = false;
+    }
+    if (!ok) return;
+    i = min(i, S.nt[q] - 1);
+    const T t = t0 + dt * (T)(i - 1);
+    T h = min(T(1), (tm - t) / dt);
+    const T *hs = S.hist + S.off[q];
+    h = i == 0 ? h * hs[0] : (T(1) - h) * hs[i - 1] + h * hs[i];
+    for (int l = i1[2]; l <= i2[2]; l++)
+        for (int k = i1[1]; k <= i2[1]; k++)
+            for (int j = i1[0]; j <= i2[0]; j++) {
+                const T w = h * ((T(1) - fabs(xi[0] - (T)j)) * (T(1) - fabs(xi[1] - (T)k)) *
+                                 (T(1) - fabs(xi[2] - (T)l)));
+                if (S.kind == 3) {
+                    const long o = P.g.at(j, k, l);
+                    for (int c = 0; c < 3; c++) atomicAdd(&P.a[c][o], w * S.w[c][q]);
+                } else {
+                    const long o = (j - P.fs_lo[0]) + (long)P.fs_n[0] * ((k - P.fs_lo[1]) + (long)P.fs_n[1] * (l - P.fs_lo[2]));
+                    for (int c = 0; c < 6; c++) atomicAdd(&P.fs[c][o], -(w * S.w[c][q]));
+                }
+            }
+}
+
+template <class T>
+void launch_finite_source(const FSrc<T> &S, T tm, const Params<T> &P, hipStream_t s) {
+    if (S.n > 0) hipLaunchKernelGGL((k_finite_source<T>), dim3((S.n + 255) / 256), dim3(256), 0, s, S, tm, P);
+}
+
 template <class T>
 void launch_stress(const Params<T> &P, const PointOps<T> &ops, int3 lo, int3 hi, int mode, hipStream_t s) {
     if (lo.x > hi.x || lo.y > hi.y || lo.z > hi.z) return;
@@ -985,6 +1045,7 @@ void launch_fault_sa(const Params<T> &P, const Fault<T> &F, hipStream_t s) {
 
 #define INST(T)                                                                                            \
     template void launch_stress<T>(const Params<T> &, const PointOps<T> &, int3, int3, int, hipStream_t);   \
+    template void launch_finite_source<T>(const FSrc<T> &, T, const Params<T> &, hipStream_t);       \
     template void launch_accel<T>(const Params<T> &, int3, int3, int, hipStream_t);                         \
     template void launch_time<T>(const Params<T> &, int3, int3, hipStream_t);                               \
     template void launch_newton<T>(const Params<T> &, int3, int3, hipStream_t);                             \

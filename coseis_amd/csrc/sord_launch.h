@@ -4,6 +4,7 @@
 #include "sord_dev.h"
 
 namespace sord {
+template <class T> void launch_finite_source(const FSrc<T> &S, T tm, const Params<T> &P, hipStream_t s);
 template <class T> void launch_stress(const Params<T> &P, const PointOps<T> &ops, int3 lo, int3 hi, int mode, hipStream_t s);
 template <class T> void launch_accel(const Params<T> &P, int3 lo, int3 hi, int mode, hipStream_t s);
 template <class T> void launch_time(const Params<T> &P, int3 lo, int3 hi, hipStream_t s);

@@ -23,9 +23,9 @@
  * field-I/O ops '=', '+', '*', '.', '=<', '+<', '*<', '=>', '.>'.  Not
  * restated: random ops ('~', and gridnoise, whose values depend on the
  * Fortran RNG), '=@' cube ops (reference defect D9), '.<' (reads a buffer row
- * with a stale loop index, sord/input_output.f90:350), strided disk input,
- * the finite kinematic source (nsource != 0), and debug dumps.  Those abort
- * with exit status 3.
+ * with a stale loop index, sord/input_output.f90:350), strided disk input
+ * and debug dumps.  Those abort with exit status 3.  The finite kinematic
+ * source (sord/kinematic_source.f90) is restated.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1642,6 +1642,109 @@ static void vnorm3(A3 f, real *t) { /* f = sqrt(sum(t*t, 4)) on the fault plane 
 /* ------------------------------------------------------------------------ */
 /* step_time: sord/time_integration.f90:6-115                               */
 /* ------------------------------------------------------------------------ */
+/* ---- finite kinematic source: sord/kinematic_source.f90 ---- */
+static int src_n = 0, *src_nt;
+static real *src_xi, *src_dt, *src_t0, *src_w1, *src_w2, *src_history; /* [isrc + n * comp] */
+
+static void iio_read(const char *fname, int *buf, int n) { /* sord/fortran_io.f90 iio1 (int32) */
+    FILE *fp = fopen(fname, "rb");
+    if (!fp) {
+        fprintf(stderr, "cannot open input file %s\n", fname);
+        exit(2);
+    }
+    if (fread(buf, sizeof(int), (size_t)n, fp) != (size_t)n) die(2, "short read");
+    fclose(fp);
+}
+
+/* sord/kinematic_source.f90:10-64 */
+static void init_finite_source(void) {
+    if (nsource == 0) return;
+    int n = abs(nsource);
+    src_xi = ralloc((size_t)n * 3);
+    rio_read("source-xi1.bin", src_xi, (size_t)n, 0, 1);
+    rio_read("source-xi2.bin", src_xi + n, (size_t)n, 0, 1);
+    rio_read("source-xi3.bin", src_xi + 2 * n, (size_t)n, 0, 1);
+    const int force = !strcmp(source, "force");
+    for (int i = 0; i < 3; i++) {
+        int below = 1, above = 1;
+        for (int q = 0; q < n; q++) {
+            real x = src_xi[q + n * i];
+            x = force ? x - (real)nnoff[i] : x - R(0.5) - (real)nnoff[i];
+            src_xi[q + n * i] = x;
+            if (!(x < R(-1.0) + (real)(force ? i1node[i] : i1cell[i]))) below = 0;
+            if (!(x > R(1.0) + (real)(force ? i2node[i] : i2cell[i]))) above = 0;
+        }
+        if (below || above) {
+            nsource = 0;
+            return;
+        }
+    }
+    src_w1 = ralloc((size_t)n * 3);
+    static const char *f1[3] = {"source-w11.bin", "source-w12.bin", "source-w13.bin"};
+    static const char *t1[3] = {"source-w11.bin", "source-w22.bin", "source-w33.bin"};
+    static const char *t2[3] = {"source-w23.bin", "source-w31.bin", "source-w12.bin"};
+    for (int i = 0; i < 3; i++) rio_read(force ? f1[i] : t1[i], src_w1 + n * i, (size_t)n, 0, 1);
+    if (!force) {
+        src_w2 = ralloc((size_t)n * 3);
+        for (int i = 0; i < 3; i++) rio_read(t2[i], src_w2 + n * i, (size_t)n, 0, 1);
+    }
+    src_t0 = ralloc((size_t)n);
+    src_dt = ralloc((size_t)n);
+    src_nt = (int *)malloc(sizeof(int) * (size_t)n);
+    rio_read("source-t0.bin", src_t0, (size_t)n, 0, 1);
+    rio_read("source-dt.bin", src_dt, (size_t)n, 0, 1);
+    iio_read("source-nt.bin", src_nt, n);
+    size_t nh = 0;
+    for (int q = 0; q < n; q++) nh += (size_t)src_nt[q];
+    src_history = ralloc(nh);
+    rio_read("source-history.bin", src_history, nh, 0, 1);
+    src_n = n;
+}
+
+/* sord/kinematic_source.f90:67-118: add the source to the force (w1) or to
+ * the potency / moment tensor (w1, w2) */
+static void finite_source(void) {
+    if (nsource == 0) return;
+    const int n = src_n, force = !strcmp(source, "force");
+    size_t itoff = 0;
+    for (int isrc = 0; isrc < n; isrc++) {
+        int i = (int)floor((double)((tm - src_t0[isrc]) / src_dt[isrc])) + 1;
+        real xi[3];
+        int i1[3], i2[3], ok = 1;
+        for (int a = 0; a < 3; a++) {
+            xi[a] = src_xi[isrc + n * a];
+            int fl = (int)floor((double)xi[a]);
+            i1[a] = imax(force ? i1node[a] : i1cell[a], fl);
+            i2[a] = imin(force ? i2node[a] : i2cell[a], fl + 1);
+            if (i2[a] < i1[a]) ok = 0;
+        }
+        if (i >= 0 && ok) {
+            i = imin(i, src_nt[isrc] - 1);
+            real t = src_t0[isrc] + src_dt[isrc] * (real)(i - 1);
+            real h = fminr(R(1.0), (tm - t) / src_dt[isrc]);
+            if (i == 0) h = h * src_history[itoff];
+            else h = (R(1.0) - h) * src_history[itoff + (size_t)i - 1] + h * src_history[itoff + (size_t)i];
+            for (int l = i1[2]; l <= i2[2]; l++)
+                for (int k = i1[1]; k <= i2[1]; k++)
+                    for (int j = i1[0]; j <= i2[0]; j++) {
+                        real w = h * ((R(1.0) - FABS(xi[0] - (real)j)) * (R(1.0) - FABS(xi[1] - (real)k)) *
+                                      (R(1.0) - FABS(xi[2] - (real)l)));
+                        for (int c = 1; c <= 3; c++) {
+                            A3 a = comp(w1, c);
+                            if (force) {
+                                AT(a, j, k, l) = AT(a, j, k, l) + w * src_w1[isrc + n * (c - 1)];
+                            } else {
+                                A3 b = comp(w2, c);
+                                AT(a, j, k, l) = AT(a, j, k, l) - w * src_w1[isrc + n * (c - 1)];
+                                AT(b, j, k, l) = AT(b, j, k, l) - w * src_w2[isrc + n * (c - 1)];
+                            }
+                        }
+                    }
+        }
+        itoff += (size_t)src_nt[isrc];
+    }
+}
+
 static void step_time(void) {
     size_t nf = (size_t)fm[0] * fm[1] * fm[2];
     int one[3] = {1, 1, 1};
@@ -1789,7 +1892,7 @@ static void step_stress(void) {
                 memcpy(w1 + (ic - 1) * vol_n, s1.p, vol_n * sizeof(real));
             }
         }
-    if (!strcmp(source, "potency") && nsource != 0) die(3, "oracle: finite source not restated");
+    if (!strcmp(source, "potency")) finite_source();
     field_io("<>", "pxx", comp(w1, 1));
     field_io("<>", "pyy", comp(w1, 2));
     field_io("<>", "pzz", comp(w1, 3));
@@ -1810,7 +1913,7 @@ static void step_stress(void) {
             b[q] = R(2.0) * mu.p[q] * b[q];
         }
     }
-    if (!strcmp(source, "moment") && nsource != 0) die(3, "oracle: finite source not restated");
+    if (!strcmp(source, "moment")) finite_source();
     field_io("<>", "mxx", comp(w1, 1));
     field_io("<>", "myy", comp(w1, 2));
     field_io("<>", "mzz", comp(w1, 3));
@@ -2079,7 +2182,7 @@ static void step_accel(void) {
                 for (size_t q = 0; q < vol_n; q++) W[q] = W[q] - s2.p[q];
             }
     }
-    if (!strcmp(source, "force") && nsource != 0) die(3, "oracle: finite source not restated");
+    if (!strcmp(source, "force")) finite_source();
     field_io("<", "fx", comp(w1, 1));
     field_io("<", "fy", comp(w1, 2));
     field_io("<", "fz", comp(w1, 3));
@@ -2228,7 +2331,7 @@ int main(void) {
     init_grid();
     init_material();
     init_pml();
-    if (nsource != 0) die(3, "oracle: finite source not restated");
+    init_finite_source();
     init_rupture();
     resample_material();
     struct timespec ta, tb;

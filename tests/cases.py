@@ -150,6 +150,49 @@ def sources_scattered():
     return prm, {}
 
 
+def _finite(kind):
+    """A finite kinematic source (sord/kinematic_source.f90) of seven
+    subfaults, some sharing cells, with their own start times, sampling
+    intervals and history lengths (one starts before t = 0, one after the
+    last step); free top, PML elsewhere."""
+    nx, ny, nz = 18, 16, 14
+    prm = {
+        'shape': [nx, ny, nz, 30], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['pml', 'pml', 'free'], 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.02],
+        'hourglass': [1.0, 1.0], 'itstats': 5,
+        'source': kind, 'nsource': 7,
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin'), ([[], [], 0, []], '=>', 'vz-top.bin')],
+    }
+    rng = np.random.default_rng({'potency': 11, 'moment': 12, 'force': 13}[kind])
+    xi = np.array([[8.3, 7.6, 6.2], [8.8, 7.6, 6.2], [8.3, 8.1, 6.7], [11.5, 5.5, 9.5],
+                   [5.0, 10.0, 4.0], [6.7, 6.2, 8.9], [12.2, 11.4, 5.6]])
+    t0 = np.array([0.0, 0.01, -0.02, 0.03, 0.005, 0.0, 0.5])
+    dts = np.array([0.02, 0.015, 0.03, 0.01, 0.025, 0.05, 0.02])
+    nts = np.array([5, 3, 6, 4, 2, 7, 3], dtype=np.int32)
+    hist = rng.uniform(0.2, 1.0, int(nts.sum()))
+    files = {'source-xi%d.bin' % (i + 1): xi[:, i] for i in range(3)}
+    files.update({'source-t0.bin': t0, 'source-dt.bin': dts, 'source-nt.bin': nts, 'source-history.bin': hist})
+    scale = {'potency': 1e-2, 'moment': 1e15, 'force': 1e10}[kind]
+    names = ['w11', 'w12', 'w13'] if kind == 'force' else ['w11', 'w22', 'w33', 'w23', 'w31', 'w12']
+    for n in names:
+        files['source-%s.bin' % n] = scale * rng.uniform(-1, 1, 7)
+    return prm, files
+
+
+def finite_potency():
+    return _finite('potency')
+
+
+def finite_moment():
+    return _finite('moment')
+
+
+def finite_force():
+    return _finite('force')
+
+
 CASES = {
     'explosion_pml': explosion_pml,
     'layered_free': layered_free,
@@ -157,10 +200,15 @@ CASES = {
     'stages': stages,
     'bc_mix': bc_mix,
     'sources_scattered': sources_scattered,
+    'finite_potency': finite_potency,
+    'finite_moment': finite_moment,
+    'finite_force': finite_force,
 }
 
 
 def write_inputs(files, realsize=4):
+    """Input files: reals in the run's precision, integer arrays as int32."""
     dt = '<f%d' % realsize
     for name, a in files.items():
-        np.asarray(a, dtype=dt).T.tofile(name)
+        a = np.asarray(a)
+        a.astype('<i4' if np.issubdtype(a.dtype, np.integer) else dt).T.tofile(name)
