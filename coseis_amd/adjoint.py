@@ -17,6 +17,7 @@ import math
 import os
 import shutil
 import tempfile
+import weakref
 
 import numpy as np
 import torch
@@ -55,6 +56,9 @@ class Problem(object):
                 prm[name] = [([[], [], []], '=<', name + '.bin')]
             text = sord.stage(prm, realsize=self.realsize)[0]
             s = solver.Solver(text, realsize=self.realsize, device=self.device)
+            # the scratch directory goes with the solver (a graph whose backward
+            # never runs frees both when it is dropped)
+            weakref.finalize(s, shutil.rmtree, d, True)
             s.adjoint_setup(self.receivers, self.every)
         except Exception:
             os.chdir(cwd)
