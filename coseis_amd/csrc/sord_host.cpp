@@ -545,10 +545,20 @@ struct Solver {
         const int it1 = o.ii[3][0], it2 = o.ii[3][1], dit = o.ii[3][2];
         if (o.op[0] == '.') fail(SORD_EUNSUPPORTED, "'.<' (the reference reads a stale buffer row, "
                                                     "sord/input_output.f90:350)");
-        for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++) {
+            // the reference interpolates a broadcast (one-value) axis read at a
+            // stride from unread scratch values, and the last stretch of a range
+            // whose stop is off the stride grid likewise
+            if (x.di[i] > 1 && (x.m[i] == 1 || (o.ii[i][1] - o.ii[i][0]) % x.di[i] != 0))
+                fail(SORD_EUNSUPPORTED, "strided disk input whose range is not a whole number of strides");
             if (x.m[i] == 1) x.i1[i] = x.i2[i] = 1, x.n[i] = 1, x.o[i] = 0;
-        for (int i = 0; i < 3; i++)
-            if (x.di[i] > 1) fail(SORD_EUNSUPPORTED, "strided disk input (interpolate)");
+        }
+        // strided input is interpolated between the read nodes
+        // (sord/utilities.f90:168-221); its k pass steps the l planes by di(1),
+        // so with di(1) != di(3) planes between the read ones would keep stale
+        // values of the reference's scratch array
+        if (x.di[1] > 1 && x.di[0] != x.di[2] && x.i2[2] > x.i1[2])
+            fail(SORD_EUNSUPPORTED, "strided disk input with di(1) != di(3) (sord/utilities.f90:197)");
         size_t rec = (size_t)x.n[0] * x.n[1] * x.n[2];
         if (o.ib < 0) {
             o.hbuf.assign(rec * o.nb, T(0));
@@ -567,6 +577,43 @@ struct Solver {
         }
         o.ib++;
         const T *b = o.hbuf.data() + rec * (size_t)(o.ib - 1);
+        std::vector<T> dense;
+        if (x.di[0] > 1 || x.di[1] > 1 || x.di[2] > 1) {
+            // the read nodes at their places in the box x.i1..x.i2, then linear
+            // interpolation along j, k, l in the reference's order and weights
+            int e[3];
+            for (int i = 0; i < 3; i++) e[i] = x.i2[i] - x.i1[i] + 1;
+            dense.assign((size_t)e[0] * e[1] * e[2], T(0));
+            auto at = [&](int j, int k, int l) -> T & { return dense[(size_t)j + (size_t)e[0] * ((size_t)k + (size_t)e[1] * l)]; };
+            for (int l = 0; l < x.n[2]; l++)
+                for (int k = 0; k < x.n[1]; k++)
+                    for (int j = 0; j < x.n[0]; j++)
+                        at(j * x.di[0], k * x.di[1], l * x.di[2]) =
+                            b[(size_t)j + (size_t)x.n[0] * ((size_t)k + (size_t)x.n[1] * l)];
+            int d = x.di[0];
+            for (int i = 1; i < d; i++) {
+                const T h1 = T(1) / (T)d * (T)i, h2 = T(1) / (T)d * (T)(d - i);
+                for (int l = 0; l < e[2]; l += x.di[2])
+                    for (int k = 0; k < e[1]; k += x.di[1])
+                        for (int j = 0; j + d < e[0]; j += d) at(j + i, k, l) = h1 * at(j, k, l) + h2 * at(j + d, k, l);
+            }
+            d = x.di[1];
+            for (int i = 1; i < d; i++) {
+                const T h1 = T(1) / (T)d * (T)i, h2 = T(1) / (T)d * (T)(d - i);
+                for (int l = 0; l < e[2]; l += x.di[0])
+                    for (int k = 0; k + d < e[1]; k += d)
+                        for (int j = 0; j < e[0]; j++) at(j, k + i, l) = h1 * at(j, k, l) + h2 * at(j, k + d, l);
+            }
+            d = x.di[2];
+            for (int i = 1; i < d; i++) {
+                const T h1 = T(1) / (T)d * (T)i, h2 = T(1) / (T)d * (T)(d - i);
+                for (int l = 0; l + d < e[2]; l += d)
+                    for (int k = 0; k < e[1]; k++)
+                        for (int j = 0; j < e[0]; j++) at(j, k, l + i) = h1 * at(j, k, l) + h2 * at(j, k, l + d);
+            }
+            b = dense.data();
+            for (int i = 0; i < 3; i++) x.n[i] = e[i];
+        }
         for (int i = 0; i < 3; i++) {
             lo[i] = x.i1[i];
             hi[i] = x.m[i] == 1 ? fn[i] : x.i2[i];

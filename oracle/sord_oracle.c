@@ -23,8 +23,7 @@
  * field-I/O ops '=', '+', '*', '.', '=<', '+<', '*<', '=>', '.>'.  Not
  * restated: random ops ('~', and gridnoise, whose values depend on the
  * Fortran RNG), '=@' cube ops (reference defect D9), '.<' (reads a buffer row
- * with a stale loop index, sord/input_output.f90:350), strided disk input
- * and debug dumps.  Those abort with exit status 3.  The finite kinematic
+ * with a stale loop index, sord/input_output.f90:350) and debug dumps.  Those abort with exit status 3.  The finite kinematic
  * source (sord/kinematic_source.f90) is restated.
  */
 #include <math.h>
@@ -846,6 +845,38 @@ static void setup_dimensions(void) {
 /* ------------------------------------------------------------------------ */
 /* field i/o: sord/input_output.f90:103-542                                 */
 /* ------------------------------------------------------------------------ */
+/* sord/utilities.f90:168-221: linear interpolation between nodes read at
+ * strides di (the k pass steps the l planes by di(1), as the reference does) */
+static void interpolate(A3 f, const int i3[3], const int i4[3], const int di[3]) {
+    int i1[3], i2[3];
+    for (int a = 0; a < 3; a++) {
+        i1[a] = i3[a], i2[a] = i4[a];
+        if (i1[a] < 1) i1[a] = i1[a] + (-i1[a] / di[a] + 1) * di[a];
+        if (i2[a] > f.n[a]) i2[a] = i1[a] + (f.n[a] - i1[a]) / di[a] * di[a];
+    }
+    int d = di[0];
+    for (int i = 1; i <= d - 1; i++) {
+        real h1 = R(1.0) / (real)d * (real)i, h2 = R(1.0) / (real)d * (real)(d - i);
+        for (int l = i1[2]; l <= i2[2]; l += di[2])
+            for (int k = i1[1]; k <= i2[1]; k += di[1])
+                for (int j = i1[0]; j <= i2[0] - d; j += d) AT(f, j + i, k, l) = h1 * AT(f, j, k, l) + h2 * AT(f, j + d, k, l);
+    }
+    d = di[1];
+    for (int i = 1; i <= d - 1; i++) {
+        real h1 = R(1.0) / (real)d * (real)i, h2 = R(1.0) / (real)d * (real)(d - i);
+        for (int l = i1[2]; l <= i2[2]; l += di[0])
+            for (int k = i1[1]; k <= i2[1] - d; k += d)
+                for (int j = i1[0]; j <= i2[0]; j++) AT(f, j, k + i, l) = h1 * AT(f, j, k, l) + h2 * AT(f, j, k + d, l);
+    }
+    d = di[2];
+    for (int i = 1; i <= d - 1; i++) {
+        real h1 = R(1.0) / (real)d * (real)i, h2 = R(1.0) / (real)d * (real)(d - i);
+        for (int l = i1[2]; l <= i2[2] - d; l += d)
+            for (int k = i1[1]; k <= i2[1]; k++)
+                for (int j = i1[0]; j <= i2[0]; j++) AT(f, j, k, l + i) = h1 * AT(f, j, k, l) + h2 * AT(f, j, k, l + d);
+    }
+}
+
 static void rio_write(const char *fname, const real *buf, size_t reclen, int o, int n) {
     FILE *fp = fopen(fname, o > 0 ? "r+b" : "wb");
     if (!fp) die(2, "cannot open output file");
@@ -1002,7 +1033,6 @@ static void field_io(const char *passes, const char *field, A3 f) {
                 } else {
                     die(3, "oracle: '.<' is not restated (reference defect)");
                 }
-                if (di[0] > 1 || di[1] > 1 || di[2] > 1) die(3, "oracle: strided disk input not restated");
                 size_t rec = (size_t)n[0] * n[1] * n[2];
                 if (io->ib < 0) {
                     io->buff = ralloc(rec * io->nb);
@@ -1034,6 +1064,11 @@ static void field_io(const char *passes, const char *field, A3 f) {
                 for (int l = i1[2]; l <= i2[2]; l += di[2])
                     for (int k = i1[1]; k <= i2[1]; k += di[1])
                         for (int j = i1[0]; j <= i2[0]; j += di[0]) AT(s1, j, k, l) = b[c++];
+                if (di[0] > 1 || di[1] > 1 || di[2] > 1) {
+                    int i3[3], i4[3];
+                    for (int a = 0; a < 3; a++) i3[a] = io->ii[a][0] - nnoff[a], i4[a] = io->ii[a][1] - nnoff[a];
+                    interpolate(s1, i3, i4, di);
+                }
                 /* half-dimension fill */
                 if (m[0] == 1) {
                     i2[0] = f.n[0];

@@ -193,6 +193,32 @@ def finite_force():
     return _finite('force')
 
 
+def strided_input():
+    """Material read at strides and interpolated between the read cells
+    (sord/utilities.f90:168-221): rho '=<' every 2 cells, vs '=<' every 3
+    cells in x and z, vp '+<' every 2 cells from an offset; ranges that are a
+    whole number of strides (elsewhere the reference reads stale values)."""
+    nx, ny, nz = 17, 15, 13
+    prm = {
+        'shape': [nx, ny, nz, 30], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2670.0, ([[0, 15, 2], [0, 13, 2], [0, 11, 2]], '=<', 'in-rho.bin')],
+        'vp': [6000.0, ([[2, 15, 2], [1, 14, 2], [0, 11, 2]], '+<', 'in-vp.bin')],
+        'vs': [3464.0, ([[0, 16, 3], [], [0, 10, 3]], '=<', 'in-vs.bin')],
+        'gam': [0.0], 'itstats': 5,
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin')],
+    }
+    prm.update(_src(['mxx', 'myy', 'mzz'], 8.0, 1e15, tau=0.03))
+    rng = np.random.default_rng(21)
+    files = {
+        'in-rho.bin': 2670.0 * (1 + 0.05 * rng.uniform(-1, 1, (8, 7, 6))),
+        'in-vp.bin': 100.0 * rng.uniform(-1, 1, (7, 7, 6)),
+        'in-vs.bin': 3464.0 * (1 + 0.05 * rng.uniform(-1, 1, (6, 14, 4))),
+    }
+    return prm, files
+
+
 CASES = {
     'explosion_pml': explosion_pml,
     'layered_free': layered_free,
@@ -203,6 +229,7 @@ CASES = {
     'finite_potency': finite_potency,
     'finite_moment': finite_moment,
     'finite_force': finite_force,
+    'strided_input': strided_input,
 }
 
 
