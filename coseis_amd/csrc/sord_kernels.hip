@@ -107,6 +107,7 @@ __device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, i
     }
     const T wx = P.hx[j], wy = P.hy[k], wz = P.hz[l];
     const T coef[3] = {wy * wz * T(0.25), wx * wz * T(0.25), wx * wy * T(0.25)};
+    const long oc = g.at(j, k, l);
     T e[3][3];
 #pragma unroll
     for (int ic = 0; ic < 3; ic++) {
@@ -117,20 +118,28 @@ __device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, i
             for (int a = 0; a < 3; a++)
                 if (a != id && inS[a]) other = true;
             const bool useV = inS[id], useU = !inS[id] && other;
-            // edge differences along id, over the 4 edges (other two corner bits)
-            const int bit = 1 << id;
-            T s = T(0);
-            int first = 1;
+            // edge differences along index axis m, over the 4 edges (other two corner bits)
+            auto edges = [&](int m) {
+                const int bit = 1 << m;
+                T s = T(0);
+                int first = 1;
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                if (q & bit) continue;
-                const T fp = useV ? V[ic][q | bit] : useU ? U[ic][q | bit] : W[ic][q | bit];
-                const T fm = useV ? V[ic][q] : useU ? U[ic][q] : W[ic][q];
-                const T d = fp - fm;
-                s = first ? d : s + d;
-                first = 0;
+                for (int q = 0; q < 8; q++) {
+                    if (q & bit) continue;
+                    const T fp = useV ? V[ic][q | bit] : useU ? U[ic][q | bit] : W[ic][q | bit];
+                    const T fm = useV ? V[ic][q] : useU ? U[ic][q] : W[ic][q];
+                    const T d = fp - fm;
+                    s = first ? d : s + d;
+                    first = 0;
+                }
+                return s;
+            };
+            T D;
+            if (!P.gen) {
+                D = coef[id] * edges(id);
+            } else {  // affine cell: the row id of the cell's operator matrix
+                D = P.cm[3 * id][oc] * edges(0) + P.cm[3 * id + 1][oc] * edges(1) + P.cm[3 * id + 2][oc] * edges(2);
             }
-            T D = coef[id] * s;
             if (inS[id]) {
                 // damped direction: g' recursion (sord/stress.f90:42-131)
                 const int a = id;
@@ -305,8 +314,23 @@ __device__ __forceinline__ void accel_node(Params<T> &P, int j, int k, int l, T 
             // diff_cn along id: sum over the 4 cell pairs (minus side, plus side)
             T s = T(0);
             int first = 1;
+            if (P.gen) {
+                // affine cells: minus the transpose of the cell difference, the
+                // node being corner (1 - d) of cell n - 1 + d
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < 8; q++) {
+                    const int d0 = q & 1, d1 = (q >> 1) & 1, d2 = (q >> 2) & 1;
+                    const long oq = g.at(j - 1 + d0, k - 1 + d1, l - 1 + d2);
+                    const T w = (d0 ? P.cm[3 * id][oq] : -P.cm[3 * id][oq]) +
+                                (d1 ? P.cm[3 * id + 1][oq] : -P.cm[3 * id + 1][oq]) +
+                                (d2 ? P.cm[3 * id + 2][oq] : -P.cm[3 * id + 2][oq]);
+                    const T t = w * S[oq];
+                    s = first ? t : s + t;
+                    first = 0;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4 && !P.gen; q++) {
                 int d[3];
                 const int b = (id + 1) % 3, cc = (id + 2) % 3;
                 d[b] = q & 1;

@@ -219,6 +219,54 @@ def strided_input():
     return prm, files
 
 
+def affine_mesh():
+    """A rotated and sheared mesh (affine transformation: parallelepiped
+    cells): two layers, free top, PML elsewhere, viscous hourglass, viscosity;
+    an explosion and a double couple."""
+    c, s = np.cos(np.radians(25.0)), np.sin(np.radians(25.0))
+    nx, ny, nz = 20, 18, 16
+    prm = {
+        'shape': [nx, ny, nz, 36], 'delta': [100.0, 100.0, 100.0, 0.006],
+        'affine': [[c, -s, 0.0], [s, c, 0.0], [0.15, 0.0, 1.0]],
+        'bc1': ['pml', 'pml', 'free'], 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2700.0, ([[], [], [0, 5]], '=', 2600.0)],
+        'vp': [6000.0, ([[], [], [0, 5]], '=', 4000.0)],
+        'vs': [3464.0, ([[], [], [0, 5]], '=', 2000.0)],
+        'gam': [0.03], 'hourglass': [1.0, 2.0], 'itstats': 6,
+        'mxy': [([9.0, 8.0, 7.0, []], '.', 1e16, 'brune', 0.04)],
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin'), ([[2, 18, 4], [2, 16, 4], 0, []], '=>', 'rcv-vz.bin')],
+        'wxy': [([[], [], 7, 20], '=>', 'wxy.bin')],
+    }
+    prm.update(_src(['mxx', 'myy', 'mzz'], 11.0, 5e15, tau=0.03))
+    return prm, {}
+
+
+def affine_expand():
+    """An expanded grid (cells growing by 1.1 from node 5 on) under a shear:
+    parallelepiped cells of varying size, PML on all faces."""
+    nx, ny, nz = 18, 16, 15
+    prm = {
+        'shape': [nx, ny, nz, 30], 'delta': [100.0, 100.0, 100.0, 0.006],
+        'affine': [[1.0, 0.2, 0.0], [0.0, 1.0, 0.0], [0.0, 0.1, 1.0]],
+        'rexpand': 1.1, 'n1expand': [0, 0, 0], 'n2expand': [5, 4, 4],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0], 'itstats': 5,
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vy': [([[], [], [], -1], '=>', 'vy.bin')],
+    }
+    prm.update(_src(['mxx', 'myy', 'mzz'], 7.0, 1e16, tau=0.03))
+    return prm, {}
+
+
+def fault_affine():
+    """The slip-weakening rupture of fault_sw on a sheared mesh (fault normals
+    and areas from the parallelepiped cells)."""
+    prm, files = fault_sw()
+    prm['affine'] = [[1.0, 0.1, 0.0], [0.0, 1.0, 0.0], [0.05, 0.0, 1.0]]
+    return prm, files
+
+
 CASES = {
     'explosion_pml': explosion_pml,
     'layered_free': layered_free,
@@ -230,6 +278,9 @@ CASES = {
     'finite_moment': finite_moment,
     'finite_force': finite_force,
     'strided_input': strided_input,
+    'affine_mesh': affine_mesh,
+    'affine_expand': affine_expand,
+    'fault_affine': fault_affine,
 }
 
 
