@@ -2,7 +2,7 @@
 
 Tolerances: the HIP kernels use the rectilinear form of the support operators
 (differences first), exact box volumes and fused multiply-adds, so FP32 results
-differ from the reference's FP32 by rounding only; the bar is the north-star
+differ from the reference's FP32 by rounding only; the bar is the target
 FP32 relative L2 error of 1e-4 on every output, and 1e-9 for FP64.
 """
 import numpy as np
