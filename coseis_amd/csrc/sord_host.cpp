@@ -1025,6 +1025,17 @@ extern "C" int sord_batch_create(int n, const char *const *texts, const char *co
     return SORD_OK;
 }
 
+extern "C" int sord_batch_set_graph(void *batch, int on, int *in_use) {
+    return dispatch_batch(batch, [&](auto &B) {
+        if (on >= 0) {
+            if (B.initialized) HIPCHECK(hipStreamSynchronize(B.stream));
+            B.drop_graphs();
+            B.use_graph = on != 0;
+        }
+        if (in_use) *in_use = B.use_graph ? 1 : 0;
+    });
+}
+
 extern "C" int sord_batch_set_tiles(void *batch, int index, int planes, int planes_boundary) {
     return dispatch_batch(batch, [&](auto &B) {
         if (index < -1 || index >= (int)B.S.size()) fail(SORD_EINPUT, "batch: no such scenario");

@@ -18,7 +18,7 @@ ERRORS = {
 EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field',
     'sord_bench', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_copy_probe',
-    'sord_batch_create', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
+    'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
     'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
     'sord_adjoint_setup', 'sord_adjoint_forward', 'sord_adjoint_backward', 'sord_adjoint_timing',
     'sord_last_error', 'sord_destroy',
@@ -64,6 +64,7 @@ def load():
     ccp = ctypes.POINTER(ctypes.c_char_p)
     lib.sord_batch_create.argtypes = [ctypes.c_int, ccp, ccp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     lib.sord_batch_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.sord_batch_set_graph.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.sord_batch_init.argtypes = [vp]
     lib.sord_batch_step.argtypes = [vp, ctypes.c_int]
     lib.sord_batch_run.argtypes = [vp]
@@ -223,6 +224,13 @@ class Batch(object):
             for q, (p, pb) in enumerate(tiles):
                 _check(lib.sord_batch_set_tiles(self._h, q, p, pb))
         _check(lib.sord_batch_init(self._h))
+
+    def set_graph(self, on=True):
+        """Replay each step's merged launches from HIP graphs (on) or launch
+        them directly (off, the default); returns whether graphs are in use."""
+        u = ctypes.c_int()
+        _check(_lib.sord_batch_set_graph(self._h, -1 if on is None else int(bool(on)), ctypes.byref(u)))
+        return bool(u.value)
 
     def set_tiles(self, index, planes, planes_boundary):
         """Planes per work item of one scenario; call replan() to apply."""

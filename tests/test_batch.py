@@ -41,7 +41,10 @@ def _outputs(d, realsize):
     return out
 
 
-def test_batch_matches_separate_runs():
+@pytest.mark.parametrize('graph', [False, True])
+def test_batch_matches_separate_runs(graph):
+    """graph: the merged launches replayed from HIP graphs (each ring slot's
+    graph replayed many times, with the scenarios' own work between)."""
     from coseis_amd import solver
     realsize = 8
     base = tempfile.mkdtemp(prefix='sord-batch-')
@@ -61,7 +64,10 @@ def test_batch_matches_separate_runs():
             texts.append(text)
             dirs.append(d)
         with solver.Batch(texts, dirs, realsize=realsize) as b:
+            if graph:
+                assert b.set_graph(True)
             b.run()
+            assert b.set_graph(None) == graph  # the captures succeeded
         for c, ds, db in zip(NAMES, single, dirs):
             a, o = _outputs(ds, realsize), _outputs(db, realsize)
             assert sorted(a) == sorted(o), c
