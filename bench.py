@@ -94,6 +94,7 @@ def parse():
     p.add_argument('--tiles', type=int, nargs=3, default=None, metavar=('W', 'NZ', 'NZB'),
                    help='tile shape: interior waves, planes interior/boundary')
     p.add_argument('--no-batch', action='store_true', help='skip the batched-ensemble comparison')
+    p.add_argument('--no-alone', action='store_true', help='skip roofline.alone (the single-stream steps)')
     p.add_argument('--batch-scenarios', type=int, default=64)
     p.add_argument('--batch-steps', type=int, default=100)
     p.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient leg')
@@ -286,7 +287,7 @@ def main():
         sys.exit(2)
     import torch
     from coseis_amd import configs, solver, sord
-    alone_steps = 0 if a.stage_only else ALONE_STEPS
+    alone_steps = 0 if a.stage_only or a.no_alone else ALONE_STEPS
     nt = a.warmup + a.steps + alone_steps
     prm, files = getattr(configs, WORKLOADS[a.workload][0])(nt=nt)
     nx, ny, nz = prm['shape'][:3]
