@@ -104,11 +104,11 @@ struct Params {
     T *gam, *mr;        // node: dt * node-averaged viscosity, inverse nodal mass
     T *lam, *mu, *yy, *vc;  // cell: Lame moduli, hourglass constant, 1/volume
     T *hx, *hy, *hz;    // cell widths along each axis (rectilinear mesh), 1-based cell index
-    // affine meshes (gen = 1): per cell C[a][i] = cofactor(J)_ai / 4 (J: the
-    // cell's edge vectors along j, k, l as columns), so that
-    // diff_nc(f, a) = sum_i C[a][i] * (sum of the 4 edge differences along i)
+    // non-rectilinear meshes (gen = 1): per cell the support-operator weight
+    // of corner q (bits dj + 2 dk + 4 dl) along axis a, cw[8 a + q], so that
+    // diff_nc(f, a) = sum_q cw[8 a + q] f(q); diff_cn is minus its transpose
     int gen;
-    T *cm[9];
+    T *cw[24];
     // region bounds (1-based local)
     int i1node[3], i2node[3], i1cell[3], i2cell[3], i1pml[3], i2pml[3];
     int i1bc[3], i2bc[3];

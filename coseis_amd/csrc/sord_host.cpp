@@ -215,7 +215,7 @@ struct Solver {
     int fm[3];
     // host initial state
     HArr<T> xx[3], xc[3], vc, mr, lam, mu, gam, yy, s1, s2;
-    HArr<T> cmat[9];     // affine meshes: per-cell operator matrices (init_grid)
+    HArr<T> cwt[24];     // non-rectilinear meshes: per-cell corner weights (init_grid)
     bool general_mesh = false;
     HArr<T> rho_c;  // cell density after the limits (kept for the adjoint's chain rule)
     std::vector<T> wax[3];  // cell widths per axis (1-based cell index, size nm+1)

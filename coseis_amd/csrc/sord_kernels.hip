@@ -137,8 +137,11 @@ __device__ __forceinline__ T stress_cell(Params<T> &P, const PointOps<T> &ops, i
             T D;
             if (!P.gen) {
                 D = coef[id] * edges(id);
-            } else {  // affine cell: the row id of the cell's operator matrix
-                D = P.cm[3 * id][oc] * edges(0) + P.cm[3 * id + 1][oc] * edges(1) + P.cm[3 * id + 2][oc] * edges(2);
+            } else {  // general cell: its corner weights along id
+                D = T(0);
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    D = D + P.cw[8 * id + q][oc] * (useV ? V[ic][q] : useU ? U[ic][q] : W[ic][q]);
             }
             if (inS[id]) {
                 // damped direction: g' recursion (sord/stress.f90:42-131)
@@ -315,16 +318,12 @@ __device__ __forceinline__ void accel_node(Params<T> &P, int j, int k, int l, T 
             T s = T(0);
             int first = 1;
             if (P.gen) {
-                // affine cells: minus the transpose of the cell difference, the
-                // node being corner (1 - d) of cell n - 1 + d
+                // general cells: minus the transpose of the cell difference, the
+                // node being corner 7 - q of the cell at offset q
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
-                    const int d0 = q & 1, d1 = (q >> 1) & 1, d2 = (q >> 2) & 1;
-                    const long oq = g.at(j - 1 + d0, k - 1 + d1, l - 1 + d2);
-                    const T w = (d0 ? P.cm[3 * id][oq] : -P.cm[3 * id][oq]) +
-                                (d1 ? P.cm[3 * id + 1][oq] : -P.cm[3 * id + 1][oq]) +
-                                (d2 ? P.cm[3 * id + 2][oq] : -P.cm[3 * id + 2][oq]);
-                    const T t = w * S[oq];
+                    const long oq = g.at(j - 1 + (q & 1), k - 1 + ((q >> 1) & 1), l - 1 + ((q >> 2) & 1));
+                    const T t = -(P.cw[8 * id + 7 - q][oq] * S[oq]);
                     s = first ? t : s + t;
                     first = 0;
                 }

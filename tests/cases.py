@@ -267,6 +267,31 @@ def fault_affine():
     return prm, files
 
 
+def topo_mesh():
+    """A curvilinear mesh read from coordinate files: a free top surface with
+    relief (up to 80 m on 100 m cells) that decays with depth, and a smooth
+    lateral warp; general hexahedral cells, PML elsewhere, a double couple."""
+    nx, ny, nz = 18, 16, 14
+    j, k, l = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij')
+    h = 80.0 * np.sin(np.pi * j / (nx - 1)) * np.cos(1.5 * np.pi * k / (ny - 1))
+    x = 100.0 * j + 20.0 * np.sin(np.pi * k / (ny - 1)) * np.sin(np.pi * l / (nz - 1))
+    y = 100.0 * k + 15.0 * np.sin(2 * np.pi * j / (nx - 1)) * (1 - l / (nz - 1))
+    z = 100.0 * l + h * (1 - l / (nz - 1))
+    prm = {
+        'shape': [nx, ny, nz, 30], 'delta': [100.0, 100.0, 100.0, 0.006],
+        'bc1': ['pml', 'pml', 'free'], 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.02],
+        'hourglass': [1.0, 2.0], 'itstats': 5,
+        'x': [([[], [], []], '=<', 'x.bin')],
+        'y': [([[], [], []], '=<', 'y.bin')],
+        'z': [([[], [], []], '=<', 'z.bin')],
+        'mxy': [([8.0, 7.0, 6.0, []], '.', 1e16, 'brune', 0.04)],
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin'), ([[], [], 0, []], '=>', 'vz-top.bin')],
+    }
+    return prm, {'x.bin': x, 'y.bin': y, 'z.bin': z}
+
+
 CASES = {
     'explosion_pml': explosion_pml,
     'layered_free': layered_free,
@@ -281,6 +306,7 @@ CASES = {
     'affine_mesh': affine_mesh,
     'affine_expand': affine_expand,
     'fault_affine': fault_affine,
+    'topo_mesh': topo_mesh,
 }
 
 
