@@ -128,7 +128,7 @@ void sord_batch_destroy(void *batch);
 
 /* ---- adjoint gradient: no counterpart in the reference (the inversion
  * workload's misfit gradient with respect to the material).  The handle's
- * run must be rectilinear with uniform spacing, free-surface faces (bc 0 on
+ * mesh must be rectilinear with uniform spacing, free-surface faces (bc 0 on
  * all six), no fault, material from inputs only and moment-tensor sources
  * (mxx..mxy field-I/O '.'/'+' ops); sord_init must have been called. */
 /* Receivers: nrec nodes as 0-based (j, k, l) of the global mesh; checkpoint

@@ -101,3 +101,16 @@ def test_fp32_gradient_matches_fp64():
         err = np.linalg.norm(g4[i] - g8[i]) / np.linalg.norm(g8[i])
         print('field', i, 'rel L2 %.2e' % err, flush=True)
         assert err < 1e-3, (i, err)
+
+
+def test_refuses_what_it_does_not_differentiate():
+    """PML faces, a non-rectilinear mesh: SORD_EUNSUPPORTED, not a wrong
+    gradient."""
+    from coseis_amd import solver
+    for change in ({'bc2': ['free', 'free', 'pml'], 'npml': 3},
+                   {'affine': [[1.0, 0.1, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]}):
+        prob, mat, w = problem((10, 9, 8), 6, 8, every=3)
+        prob.prm.update(change)
+        with pytest.raises(solver.SolverError) as e:
+            gradient(prob, mat, w)
+        assert e.value.name == 'EUNSUPPORTED', e.value
