@@ -201,7 +201,9 @@ def batched_leg(nscen, steps, warmup=5, tune_steps=10):
             texts.append(sord.stage(prm)[0])
             dirs.append(d)
         os.chdir(cwd)
+        t0 = time.perf_counter()
         with solver.Batch(texts, dirs) as b:
+            init_s = time.perf_counter() - t0
             b.step(warmup)
             choice, _, trials = autotune.tune_batch(b, shapes, steps=tune_steps)
             ms_b, nu_b = b.bench(steps)
@@ -225,7 +227,7 @@ def batched_leg(nscen, steps, warmup=5, tune_steps=10):
         'batched_gnode_s': round(rb, 3), 'batched_ms_per_step': round(ms_b / steps, 4),
         'plain_sequential_gnode_s': round(rp, 3), 'plain_ms_per_step_sum': round(ms_p / steps, 4),
         'tiles_by_bucket': {str(k): list(v) for k, v in sorted(choice.items())},
-        'autotune_trials': len(trials),
+        'autotune_trials': len(trials), 'init_s': round(init_s, 2),
     }
 
 

@@ -18,6 +18,9 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <atomic>
+#include <exception>
+#include <thread>
 
 #include "sord_amd.h"
 #include "sord_launch.h"
