@@ -91,9 +91,10 @@ struct FSrc {
 template <class T>
 struct Params {
     Grid<T> g;
-    // finite source values of the current step over the box fs_lo + [0, fs_n)
-    // (cells), added to the potency (fs_kind 1) or moment (2) by the stress
-    // kernel; a force source (3) goes straight into the force
+    // finite source values of the current step over the box fs_lo + [0, fs_n):
+    // cells of the potency (fs_kind 1) or moment (2), six components, added by
+    // the stress computation; nodes of the force (3), three components, added
+    // to the force by the fused kernels (the stage path adds it straight to a)
     int fs_kind, fs_lo[3], fs_n[3];
     T *fs[6];
     T dt;

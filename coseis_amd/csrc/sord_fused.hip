@@ -84,10 +84,13 @@ __device__ __forceinline__ T tmax(T a, T b) {
 // ---------------------------------------------------------------------------
 
 // source ops on one cell in the reference's units: volume-integrated strain
-// (potency), moment, stress; G is the true gradient, returns the stress
+// (potency), moment, stress; G is the true gradient, returns the stress.  fv:
+// the finite source's values of this cell, added to the potency (fk 1) or the
+// moment (fk 2) before that stage's ops, as sord/stress.f90:170-173,215-218
+// call it.
 template <class T>
 __device__ __forceinline__ void cell_ops(const PointOps<T> &ops, unsigned boxes, long o, const T G[3][3], T lm,
-                                         T m, T vol, T rvol, T s[6]) {
+                                         T m, T vol, T rvol, int fk, const T fv[6], T s[6]) {
     T w1[3], w2[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) w1[i] = G[i][i] * vol;
@@ -110,6 +113,12 @@ __device__ __forceinline__ void cell_ops(const PointOps<T> &ops, unsigned boxes,
             }
         }
     };
+    auto add_fs = [&](int kind) {
+        if (fk != kind) return;
+#pragma unroll
+        for (int i = 0; i < 3; i++) w1[i] = w1[i] + fv[i], w2[i] = w2[i] + fv[3 + i];
+    };
+    add_fs(1);
     apply(0);
     const T tr = lm * (w1[0] + w1[1] + w1[2]);
 #pragma unroll
@@ -117,6 +126,7 @@ __device__ __forceinline__ void cell_ops(const PointOps<T> &ops, unsigned boxes,
         w1[i] = T(2) * m * w1[i] + tr;
         w2[i] = T(2) * m * w2[i];
     }
+    add_fs(2);
     apply(1);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -125,6 +135,41 @@ __device__ __forceinline__ void cell_ops(const PointOps<T> &ops, unsigned boxes,
     }
     apply(2);
     s[0] = w1[0], s[1] = w1[1], s[2] = w1[2], s[3] = w2[0], s[4] = w2[1], s[5] = w2[2];
+}
+
+// Finite source box (cells of the potency or moment, nodes of the force):
+// whether a tile's cell rows [j0, j1] x [k0, k1] (its node rows for the force)
+// meet it, uniform over the block, and the box index of one cell or node
+// (-1 outside).
+struct FsTile {
+    bool cells, nodes;
+};
+template <class T>
+__device__ __forceinline__ FsTile fs_tile(const Params<T> &P, int j0, int j1, int k0, int k1) {
+    FsTile t{false, false};
+    if (!P.fs_kind || P.fs_n[0] <= 0) return t;
+    const bool fn = P.fs_kind == 3;
+    const int jn0 = fn ? j0 + 1 : j0, kn0 = fn ? k0 + 1 : k0;  // owned nodes start one row in
+    const bool meet = jn0 < P.fs_lo[0] + P.fs_n[0] && j1 >= P.fs_lo[0] && kn0 < P.fs_lo[1] + P.fs_n[1] &&
+                      k1 >= P.fs_lo[1];
+    t.cells = meet && !fn;
+    t.nodes = meet && fn;
+    return t;
+}
+template <class T>
+__device__ __forceinline__ long fs_index(const Params<T> &P, int j, int k, int l) {
+    const unsigned dj = j - P.fs_lo[0], dk = k - P.fs_lo[1], dl = l - P.fs_lo[2];
+    if (dj >= (unsigned)P.fs_n[0] || dk >= (unsigned)P.fs_n[1] || dl >= (unsigned)P.fs_n[2]) return -1;
+    return (long)dj + (long)P.fs_n[0] * ((long)dk + (long)P.fs_n[1] * dl);
+}
+// the finite force on node (j, k, l), added before the boundary conditions as
+// sord/acceleration.f90:213-216 adds it
+template <class T>
+__device__ __forceinline__ void fs_force(const Params<T> &P, int j, int k, int l, T f[3]) {
+    const long q = fs_index(P, j, k, l);
+    if (q < 0) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) f[c] = f[c] + P.fs[c][q];
 }
 
 // the four hourglass modes of one component over the cell corners
@@ -145,7 +190,7 @@ __device__ __forceinline__ void hg_modes(T f00, T f10, T f01, T f11, T g00, T g1
 // ---------------------------------------------------------------------------
 // interior tiles
 // ---------------------------------------------------------------------------
-template <class T, int W>
+template <class T, int W, bool FS>
 __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
                                               const Work &wk) {
     __shared__ T sN[6][W][64];
@@ -158,6 +203,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
     const bool stat_now = P.stat_now, stat_next = P.stat_next;
     const unsigned tile_boxes = ops.n ? ops_boxes_jk(ops, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : 0u;
+    const FsTile fst = FS ? fs_tile(P, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : FsTile{false, false};
     T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
 
     // node data: W = u + gamma v (strain), F = h1 u + dt h2 v (hourglass)
@@ -221,7 +267,9 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
             const unsigned cl = ops_boxes_l(ops, tile_boxes, l);
             if (cl && cellv) bx = ops_boxes(ops, cl, j, k, l);
         }
-        if (!bx) {
+        // finite source (potency, moment) on this cell
+        const long fq = fst.cells && cellv ? fs_index(P, j, k, l) : -1;
+        if (!bx && fq < 0) {
             const T tr = G[0][0] + G[1][1] + G[2][2];
             const T m2 = m + m;
             s[0] = m2 * G[0][0] + lm * tr;
@@ -231,7 +279,10 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
             s[4] = m * (G[0][2] + G[2][0]);
             s[5] = m * (G[1][0] + G[0][1]);
         } else {
-            cell_ops<T>(ops, bx, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
+            T fv[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) fv[c] = fq >= 0 ? P.fs[c][fq] : T(0);
+            cell_ops<T>(ops, bx, g.at(j, k, l), G, lm, m, C.vol, C.rvol, fq >= 0 ? P.fs_kind : 0, fv, s);
         }
         if (stat_now && cellv && st)
             st_w = tmax(st_w, (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + T(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
@@ -307,6 +358,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
         }
         if (own && act) {
             const long o = g.at(j, k, l);
+            if (fst.nodes) fs_force(P, j, k, l, f);
             const T mrn = cm[3];
             T a2 = T(0), v2 = T(0), u2 = T(0);
 #pragma unroll
@@ -351,7 +403,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T
 void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;  // uniform over the block
-    interior_item<T, W>(P, C, ops, wk);
+    interior_item<T, W, false>(P, C, ops, wk);
+}
+// the same with a finite source: its box enters the stress or the force
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_fused_fs(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+    const Work wk = C.work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;
+    interior_item<T, W, true>(P, C, ops, wk);
 }
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
@@ -360,7 +420,7 @@ void k_fused_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work
     if (wk.lz0 > wk.lz1) return;
     const Scen<T> &S = sc[wk.pad1];
     if (!S.active) return;  // uniform over the block
-    interior_item<T, W>(S.P, S.C, S.ops, wk);
+    interior_item<T, W, false>(S.P, S.C, S.ops, wk);
 }
 
 // ---------------------------------------------------------------------------
@@ -372,7 +432,7 @@ void k_fused_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work
 // Rows: an item of width je-jb+1 runs LW = width+2 lanes per node row (Work.pad0)
 // and R = 64/LW rows per wavefront, so narrow items (the x slabs) fill the
 // wavefront; node rows of the block are w*R + lane/LW, with halo rows 0 and W*R-1.
-template <class T, int W, int MASK>
+template <class T, int W, int MASK, bool FS>
 __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
                                               const Work &wk) {
     constexpr int NS = W * 64 + 64;  // LDS slots per value; lanes past R*LW use the last 64
@@ -391,6 +451,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     const bool cellv = valid && jj <= LW - 2 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
     const Grid<T> &g = P.g;
     const unsigned tile_boxes = ops.n ? ops_boxes_jk(ops, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : 0u;
+    const FsTile fst = FS ? fs_tile(P, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : FsTile{false, false};
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
     const int lmax = g.nm[2];
     const bool stat_now = P.stat_now, stat_next = P.stat_next;
@@ -517,7 +578,9 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
             const unsigned cl = ops_boxes_l(ops, tile_boxes, l);
             if (cl && cellv) bx = ops_boxes(ops, cl, j, k, l);
         }
-        if (!bx) {
+        // finite source (potency, moment) on this cell
+        const long fq = fst.cells && cellv ? fs_index(P, j, k, l) : -1;
+        if (!bx && fq < 0) {
             const T tr = G[0][0] + G[1][1] + G[2][2];
             const T m2 = m + m;
             s[0] = m2 * G[0][0] + lm * tr;
@@ -527,7 +590,10 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
             s[4] = m * (G[0][2] + G[2][0]);
             s[5] = m * (G[1][0] + G[0][1]);
         } else {
-            cell_ops<T>(ops, bx, g.at(j, k, l), G, lm, m, C.vol, C.rvol, s);
+            T fv[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) fv[c] = fq >= 0 ? P.fs[c][fq] : T(0);
+            cell_ops<T>(ops, bx, g.at(j, k, l), G, lm, m, C.vol, C.rvol, fq >= 0 ? P.fs_kind : 0, fv, s);
         }
         if (!real)
 #pragma unroll
@@ -683,6 +749,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
                 if ((MASK >> a & 1) && (n3[a] < P.i1pml[a] + 1 || n3[a] > P.i2pml[a] - 1)) interior = false;
 #pragma unroll
             for (int c = 0; c < 3; c++) f[c] = interior ? f[c] + Hf[c] : (MASK && P.hg_pml) ? f[c] + Hv[c] : f[c];
+            if (fst.nodes) fs_force(P, j, k, l, f);
             // plane-local boundary conditions (sord/boundary_cond.f90:44-66)
 #pragma unroll
             for (int a = 0; a < 3; a++) {
@@ -741,7 +808,14 @@ template <class T, int W, int MASK>
 __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;  // uniform over the block
-    boundary_item<T, W, MASK>(P, C, ops, wk);
+    boundary_item<T, W, MASK, false>(P, C, ops, wk);
+}
+// a finite source: every boundary item on the kernel of all damped axes
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void k_fusedp_fs(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+    const Work wk = C.work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;
+    boundary_item<T, W, 7, true>(P, C, ops, wk);
 }
 template <class T, int W, int MASK>
 __global__ __launch_bounds__(64 * W) void k_fusedp_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work) {
@@ -749,7 +823,7 @@ __global__ __launch_bounds__(64 * W) void k_fusedp_batch(const Scen<T> *__restri
     if (wk.lz0 > wk.lz1) return;
     const Scen<T> &S = sc[wk.pad1];
     if (!S.active) return;  // uniform over the block
-    boundary_item<T, W, MASK>(S.P, S.C, S.ops, wk);
+    boundary_item<T, W, MASK, false>(S.P, S.C, S.ops, wk);
 }
 
 template <class T>
@@ -771,10 +845,17 @@ FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work) {
 
 template <class T>
 void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3],
-                  int waves, hipStream_t s) {
+                  int waves, bool fs, hipStream_t s) {
     if (nwork <= 0) return;
     const FusedC<T> C = fused_consts<T>(P, sp, work);
     const dim3 g(nwork), b8(64, 8), bb(64, BOUNDARY_WAVES);
+    if (fs) {
+        if (mask >= 0) hipLaunchKernelGGL((k_fusedp_fs<T, BOUNDARY_WAVES>), g, bb, 0, s, P, C, ops);
+        else if (waves == 4) hipLaunchKernelGGL((k_fused_fs<T, 4>), g, dim3(64, 4), 0, s, P, C, ops);
+        else if (waves == 8) hipLaunchKernelGGL((k_fused_fs<T, 8>), g, b8, 0, s, P, C, ops);
+        else hipLaunchKernelGGL((k_fused_fs<T, 16>), g, dim3(64, 16), 0, s, P, C, ops);
+        return;
+    }
     switch (mask) {
     case -1:
         switch (waves) {
@@ -832,8 +913,8 @@ template FusedC<double> fused_consts<double>(const Params<double> &, const doubl
 template void launch_fused_batch<float>(const Scen<float> *, const Work *, int, int, hipStream_t);
 template void launch_fused_batch<double>(const Scen<double> *, const Work *, int, int, hipStream_t);
 template void launch_fused<float>(const Params<float> &, const PointOps<float> &, const Work *, int, int,
-                                  const float *, int, hipStream_t);
+                                  const float *, int, bool, hipStream_t);
 template void launch_fused<double>(const Params<double> &, const PointOps<double> &, const Work *, int, int,
-                                   const double *, int, hipStream_t);
+                                   const double *, int, bool, hipStream_t);
 
 }  // namespace sord

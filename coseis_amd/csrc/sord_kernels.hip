@@ -897,10 +897,10 @@ static inline dim3 box_grid(int3 lo, int3 hi, dim3 blk) {
 // Finite kinematic source (sord/kinematic_source.f90:67-118), one thread per
 // subfault: the history value at tm (linear between samples), spread over the
 // 2 x 2 x 2 cells or nodes around the subfault with trilinear weights, added
-// to the step's source box (potency, moment: minus the tensor) or to the
-// force.  Subfaults that share a cell add atomically.
+// to the step's source box (potency, moment: minus the tensor; force with
+// box set) or to the force a.  Subfaults that share a cell add atomically.
 template <class T>
-__global__ __launch_bounds__(256) void k_finite_source(FSrc<T> S, T tm, Params<T> P) {
+__global__ __launch_bounds__(256) void k_finite_source(FSrc<T> S, T tm, Params<T> P, int box) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= S.n) return;
     const T t0 = S.t0[q], dt = S.dt[q];
@@ -927,19 +927,21 @@ __global__ __launch_bounds__(256) void k_finite_source(FSrc<T> S, T tm, Params<T
             for (int j = i1[0]; j <= i2[0]; j++) {
                 const T w = h * ((T(1) - fabs(xi[0] - (T)j)) * (T(1) - fabs(xi[1] - (T)k)) *
                                  (T(1) - fabs(xi[2] - (T)l)));
-                if (S.kind == 3) {
-                    const long o = P.g.at(j, k, l);
-                    for (int c = 0; c < 3; c++) atomicAdd(&P.a[c][o], w * S.w[c][q]);
+                const long o = (j - P.fs_lo[0]) + (long)P.fs_n[0] * ((k - P.fs_lo[1]) + (long)P.fs_n[1] * (l - P.fs_lo[2]));
+                if (S.kind == 3 && !box) {
+                    const long on = P.g.at(j, k, l);
+                    for (int c = 0; c < 3; c++) atomicAdd(&P.a[c][on], w * S.w[c][q]);
+                } else if (S.kind == 3) {
+                    for (int c = 0; c < 3; c++) atomicAdd(&P.fs[c][o], w * S.w[c][q]);
                 } else {
-                    const long o = (j - P.fs_lo[0]) + (long)P.fs_n[0] * ((k - P.fs_lo[1]) + (long)P.fs_n[1] * (l - P.fs_lo[2]));
                     for (int c = 0; c < 6; c++) atomicAdd(&P.fs[c][o], -(w * S.w[c][q]));
                 }
             }
 }
 
 template <class T>
-void launch_finite_source(const FSrc<T> &S, T tm, const Params<T> &P, hipStream_t s) {
-    if (S.n > 0) hipLaunchKernelGGL((k_finite_source<T>), dim3((S.n + 255) / 256), dim3(256), 0, s, S, tm, P);
+void launch_finite_source(const FSrc<T> &S, T tm, const Params<T> &P, bool box, hipStream_t s) {
+    if (S.n > 0) hipLaunchKernelGGL((k_finite_source<T>), dim3((S.n + 255) / 256), dim3(256), 0, s, S, tm, P, (int)box);
 }
 
 template <class T>
@@ -1068,7 +1070,7 @@ void launch_fault_sa(const Params<T> &P, const Fault<T> &F, hipStream_t s) {
 
 #define INST(T)                                                                                            \
     template void launch_stress<T>(const Params<T> &, const PointOps<T> &, int3, int3, int, hipStream_t);   \
-    template void launch_finite_source<T>(const FSrc<T> &, T, const Params<T> &, hipStream_t);       \
+    template void launch_finite_source<T>(const FSrc<T> &, T, const Params<T> &, bool, hipStream_t);       \
     template void launch_accel<T>(const Params<T> &, int3, int3, int, hipStream_t);                         \
     template void launch_time<T>(const Params<T> &, int3, int3, hipStream_t);                               \
     template void launch_newton<T>(const Params<T> &, int3, int3, hipStream_t);                             \

@@ -4,7 +4,7 @@
 #include "sord_dev.h"
 
 namespace sord {
-template <class T> void launch_finite_source(const FSrc<T> &S, T tm, const Params<T> &P, hipStream_t s);
+template <class T> void launch_finite_source(const FSrc<T> &S, T tm, const Params<T> &P, bool box, hipStream_t s);
 template <class T> void launch_stress(const Params<T> &P, const PointOps<T> &ops, int3 lo, int3 hi, int mode, hipStream_t s);
 template <class T> void launch_accel(const Params<T> &P, int3 lo, int3 hi, int mode, hipStream_t s);
 template <class T> void launch_time(const Params<T> &P, int3 lo, int3 hi, hipStream_t s);
@@ -27,7 +27,7 @@ template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, h
 namespace sord {
 // mask -1: interior kernel with `waves` waves per block; 0..7: boundary kernel
 // specialised on the damped axes (bit a: axis a), 8 waves per block
-template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, hipStream_t s);
+template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, bool fs, hipStream_t s);
 // constants of the fused kernels for one scenario; the batch launch takes a
 // device table of scenarios, each work item naming its own (Work.pad1)
 template <class T> FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work);

@@ -44,8 +44,9 @@ def collect(d, files, realsize):
     return out
 
 
-def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, side_streams=True):
-    """Stage a case in a scratch directory and run it; returns {name: array}."""
+def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, side_streams=True, probe=None):
+    """Stage a case in a scratch directory and run it; returns {name: array}.
+    probe(solver), if given, is called on the HIP solver before the run."""
     prm, files = cases.CASES[case]()
     if prm_update:
         prm.update(prm_update)
@@ -61,6 +62,8 @@ def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, 
         elif runner == 'hip':
             from coseis_amd import solver
             with solver.Solver(text, realsize=realsize, fused=fused, tiles=tiles, side_streams=side_streams) as s:
+                if probe:
+                    probe(s)
                 s.run()
         else:
             raise ValueError(runner)

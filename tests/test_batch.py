@@ -15,7 +15,7 @@ import cases
 from coseis_amd import sord
 
 pytestmark = pytest.mark.gpu
-NAMES = ['explosion_pml', 'layered_free', 'fault_sw', 'bc_mix', 'stages']
+NAMES = ['explosion_pml', 'layered_free', 'fault_sw', 'bc_mix', 'stages', 'finite_moment', 'finite_force']
 
 
 def _stage(base, case, tag, realsize):

@@ -48,15 +48,20 @@ def test_hip_matches_reference(case, realsize):
     assert not bad, bad
 
 
-@pytest.mark.parametrize('case', ['explosion_pml', 'layered_free', 'fault_sw'])
+@pytest.mark.parametrize('case', ['explosion_pml', 'layered_free', 'fault_sw', 'finite_potency', 'finite_moment',
+                                  'finite_force'])
 def test_fused_kernel_matches_stage_kernels(case):
     """The fused interior kernel (differences of true gradients) and the stage
     kernels (volume-integrated operators) are the same algebra: in FP64 they
     agree to rounding; in FP32 to the case's own FP32 rounding noise, measured
     as the distance of the FP32 stage run from the FP64 stage run (the rupture
-    case amplifies rounding through its friction law)."""
+    case amplifies rounding through its friction law).  The finite sources
+    enter the fused kernels' stress (potency, moment) or force from a box the
+    source kernel fills each step."""
+    names = []
     a64 = run_case(case, 'hip', 8, fused=False)
-    b64 = run_case(case, 'hip', 8, fused=True)
+    b64 = run_case(case, 'hip', 8, fused=True, probe=lambda s: names.append(s.bench(1)[3]))
+    assert names == ['k_fused'], names  # the fused kernels ran the steps
     for k in a64:
         assert rel_l2(b64[k], a64[k]) < 1e-9, (8, k, rel_l2(b64[k], a64[k]))
     a = run_case(case, 'hip', 4, fused=False)
