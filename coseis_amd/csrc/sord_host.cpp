@@ -255,7 +255,6 @@ struct Solver {
     bool time_pending = false;  // the leapfrog of the next step is still to do (split mode)
     bool have_sig = false;
     T *dscratch = nullptr;  // one volume array of scratch
-    T *dlam2 = nullptr, *dmu2 = nullptr;  // fused path: lam/V and 2 mu/V
     T *red_val = nullptr;
     long *red_loc = nullptr;
     bool use_fused = true;
@@ -915,6 +914,10 @@ extern "C" int sord_field(void *h, const char *name, void **ptr, int64_t shape[3
         }
         for (int i = 0; i < 6; i++)
             if (n == wn[i]) p = S.P.sig[i];
+        static const char *mn[5] = {"lam", "mu", "yy", "mr", "gam"};
+        void *mp[5] = {S.P.lam, S.P.mu, S.P.yy, S.P.mr, S.P.gam};
+        for (int i = 0; i < 5; i++)
+            if (n == mn[i]) p = mp[i];
         if (!p) fail(SORD_EINPUT, "no device field named '" + n + "'");
         HIPCHECK(hipStreamSynchronize(S.stream));
         *ptr = p;
@@ -922,6 +925,17 @@ extern "C" int sord_field(void *h, const char *name, void **ptr, int64_t shape[3
         strides[0] = 1, strides[1] = S.P.g.P0, strides[2] = S.P.g.P1;
         *offset = S.P.g.off;
     });
+}
+
+extern "C" int sord_tensor_numel(void *h, int64_t *numel) {
+    return dispatch(h, [&](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_tensor_numel before sord_init");
+        *numel = (int64_t)S.P.g.S;
+    });
+}
+
+extern "C" int sord_set_tensor(void *h, const char *name, void *dev_ptr, int64_t numel) {
+    return dispatch(h, [&](auto &S) { S.adopt(name ? name : "", dev_ptr, numel); });
 }
 
 extern "C" int sord_set_fused(void *h, int on) {

@@ -16,7 +16,8 @@ ERRORS = {
     7: 'EBADOP', 8: 'EFILE', 9: 'EHIP', 10: 'EUNSUPPORTED', 11: 'ESTATE',
 }
 EXPORTS = [
-    'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field',
+    'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field', 'sord_set_tensor',
+    'sord_tensor_numel',
     'sord_bench', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_copy_probe',
     'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
     'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
@@ -52,6 +53,8 @@ def load():
     lib.sord_status.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     i64p = ctypes.POINTER(ctypes.c_int64)
     lib.sord_field.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), i64p, i64p, i64p]
+    lib.sord_set_tensor.argtypes = [vp, ctypes.c_char_p, vp, ctypes.c_int64]
+    lib.sord_tensor_numel.argtypes = [vp, i64p]
     lib.sord_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_char_p, ctypes.c_int]
     lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
@@ -106,6 +109,7 @@ class Solver(object):
     def __init__(self, text, realsize=4, device=0, fused=True, tiles=None, side_streams=True):
         lib = load()
         self._h = ctypes.c_void_p()
+        self._adopted = {}  # caller tensors the handle uses (set_tensor)
         _check(lib.sord_create(text.encode(), realsize, device, ctypes.byref(self._h)))
         self.realsize = realsize
         if not fused:
@@ -189,10 +193,30 @@ class Solver(object):
         _check(_lib.sord_field(self._h, name.encode(), ctypes.byref(p), shp, st, ctypes.byref(off)))
         return p.value, tuple(shp), tuple(st), off.value
 
+    def tensor_numel(self):
+        """Elements of one padded solver array (what set_tensor needs)."""
+        n = ctypes.c_int64()
+        _check(_lib.sord_tensor_numel(self._h, ctypes.byref(n)))
+        return n.value
+
+    def set_tensor(self, name, tensor):
+        """Adopt a caller-owned device tensor (flat, contiguous, of the
+        solver's precision, at least tensor_numel() elements) for the named
+        array (sord_set_tensor): its current contents are copied in and the
+        kernels use it from then on.  The tensor must outlive the handle's use
+        of it; the handle keeps a reference to it."""
+        import torch
+        want = torch.float32 if self.realsize == 4 else torch.float64
+        if tensor.dtype != want or not tensor.is_cuda or not tensor.is_contiguous():
+            raise ValueError('set_tensor: a contiguous %s device tensor' % want)
+        _check(_lib.sord_set_tensor(self._h, name.encode(), ctypes.c_void_p(tensor.data_ptr()), tensor.numel()))
+        self._adopted[name] = tensor
+
     def close(self):
         if self._h:
             _lib.sord_destroy(self._h)
             self._h = None
+        self._adopted = {}
 
     def __enter__(self):
         return self

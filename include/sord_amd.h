@@ -56,14 +56,35 @@ int sord_run(void *handle);
 /* Current step (0 after init) and total steps. */
 int sord_status(void *handle, int *it, int *nt);
 
-/* Borrow a device view of a solver array (the handle keeps ownership).
- * name: "vx" "vy" "vz" "ux" "uy" "uz" (nodes), "wxx".."wxy" (cell stress of
- * the last step), "rho"... is not kept on the device.  The element (j,k,l),
- * 1-based local index over the padded mesh, is at
+/* Borrow a device view of a solver array (the handle keeps ownership; the
+ * state vectors move between two buffers each step, so a view of them holds
+ * until the next step).  name: "vx" "vy" "vz" "ux" "uy" "uz" (nodes), "wxx"..
+ * "wxy" (cell stress of the last step), "lam" "mu" "yy" (cells), "mr" "gam"
+ * (nodes); "rho", "vp", "vs" are not kept on the device.  The element
+ * (j,k,l), 1-based local index over the padded mesh, is at
  *   ptr[offset + (j-1) + strides[1]*(k-1) + strides[2]*(l-1)].
  * shape[] receives the padded extents. */
 int sord_field(void *handle, const char *name, void **dev_ptr, int64_t shape[3], int64_t strides[3],
                int64_t *offset);
+
+/* Adopt a caller-owned device buffer (on the handle's device, `numel`
+ * elements of realsize bytes, at least sord_tensor_numel's count) for a
+ * solver array, in sord_field's layout.  Replaces the reference's arrays
+ * allocated by sord/arrays.f90:6-34 for the caller's use (autograd, ensembles
+ * driven from the caller's tensors).  The current contents (halo and padding
+ * included) are copied into the buffer, which the kernels then read and
+ * write; the caller keeps ownership and must keep it alive until
+ * sord_destroy or until it adopts another.  Names: "vx" "vy" "vz" "ux" "uy"
+ * "uz" (the buffer holding the current state), the same with ".next" (the
+ * buffer the next step writes; the two exchange roles every step, and
+ * sord_field reports the current one), "lam" "mu" "yy" (cells), "mr" "gam"
+ * (nodes).  Material the caller edits between steps enters the stress, the
+ * hourglass and Newton from the next step; the fault-plane moduli and the
+ * Courant check stay those of sord_init.  Refused while the adjoint is set
+ * up, and the adjoint is refused on a handle with adopted buffers. */
+int sord_set_tensor(void *handle, const char *name, void *dev_ptr, int64_t numel);
+/* Elements of one padded solver array (the size sord_set_tensor needs). */
+int sord_tensor_numel(void *handle, int64_t *numel);
 
 /* Advance `nsteps` steps (with their field i/o) timed with HIP events on the
  * library's stream.  Returns the elapsed time of all steps (ms), the mean
