@@ -190,7 +190,7 @@ __device__ __forceinline__ void hg_modes(T f00, T f10, T f01, T f11, T g00, T g1
 // ---------------------------------------------------------------------------
 // interior tiles
 // ---------------------------------------------------------------------------
-template <class T, int W, bool FS>
+template <class T, int W, bool FS, bool ST>
 __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
                                               const Work &wk) {
     __shared__ T sN[6][W][64];
@@ -201,7 +201,9 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     const bool cellv = lane <= 62 && w <= W - 2 && j <= wk.je && k <= wk.ke;
     const Grid<T> &g = P.g;
     const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
-    const bool stat_now = P.stat_now, stat_next = P.stat_next;
+    // ST: a statistics step (its maxima add registers and work; the other
+    // steps run an instantiation without them)
+    const bool stat_now = ST && P.stat_now, stat_next = ST && P.stat_next;
     const unsigned tile_boxes = ops.n ? ops_boxes_jk(ops, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : 0u;
     const FsTile fst = FS ? fs_tile(P, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : FsTile{false, false};
     T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
@@ -397,21 +399,34 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
 }
 
 // one work item per block; the plain kernel gets one scenario in its
-// arguments, the batch kernel looks its item's scenario up (Work.pad1)
+// arguments, the batch kernel looks its item's scenario up (Work.pad1).  The
+// steps without statistics (9 in 10 by default) run instantiations without
+// the maxima: their four accumulators pushed the FP32 kernel past its 128
+// VGPRs, and the spill reloads in the plane loop waited for every load in
+// flight, the next plane's prefetch included (C2: 0.482 against 0.427 ms per
+// interior launch).
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;  // uniform over the block
-    interior_item<T, W, false>(P, C, ops, wk);
+    interior_item<T, W, false, false>(P, C, ops, wk);
 }
-// the same with a finite source: its box enters the stress or the force
+// the same on a statistics step
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_fused_stat(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+    const Work wk = C.work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;
+    interior_item<T, W, false, true>(P, C, ops, wk);
+}
+// with a finite source: its box enters the stress or the force
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 void k_fused_fs(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;
-    interior_item<T, W, true>(P, C, ops, wk);
+    interior_item<T, W, true, true>(P, C, ops, wk);
 }
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
@@ -420,7 +435,17 @@ void k_fused_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work
     if (wk.lz0 > wk.lz1) return;
     const Scen<T> &S = sc[wk.pad1];
     if (!S.active) return;  // uniform over the block
-    interior_item<T, W, false>(S.P, S.C, S.ops, wk);
+    interior_item<T, W, false, false>(S.P, S.C, S.ops, wk);
+}
+// a batch step where some merged scenario takes statistics
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_fused_batch_stat(const Scen<T> *__restrict__ sc, const Work *__restrict__ work) {
+    const Work wk = work[blockIdx.x];
+    if (wk.lz0 > wk.lz1) return;
+    const Scen<T> &S = sc[wk.pad1];
+    if (!S.active) return;  // uniform over the block
+    interior_item<T, W, false, true>(S.P, S.C, S.ops, wk);
 }
 
 // ---------------------------------------------------------------------------
@@ -856,13 +881,19 @@ void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, 
         else hipLaunchKernelGGL((k_fused_fs<T, 16>), g, dim3(64, 16), 0, s, P, C, ops);
         return;
     }
+    const bool st = P.stat_now || P.stat_next;
     switch (mask) {
     case -1:
-        switch (waves) {
-        case 4: hipLaunchKernelGGL((k_fused<T, 4>), g, dim3(64, 4), 0, s, P, C, ops); break;
-        case 8: hipLaunchKernelGGL((k_fused<T, 8>), g, b8, 0, s, P, C, ops); break;
-        default: hipLaunchKernelGGL((k_fused<T, 16>), g, dim3(64, 16), 0, s, P, C, ops); break;
-        }
+        if (st) switch (waves) {
+            case 4: hipLaunchKernelGGL((k_fused_stat<T, 4>), g, dim3(64, 4), 0, s, P, C, ops); break;
+            case 8: hipLaunchKernelGGL((k_fused_stat<T, 8>), g, b8, 0, s, P, C, ops); break;
+            default: hipLaunchKernelGGL((k_fused_stat<T, 16>), g, dim3(64, 16), 0, s, P, C, ops); break;
+            }
+        else switch (waves) {
+            case 4: hipLaunchKernelGGL((k_fused<T, 4>), g, dim3(64, 4), 0, s, P, C, ops); break;
+            case 8: hipLaunchKernelGGL((k_fused<T, 8>), g, b8, 0, s, P, C, ops); break;
+            default: hipLaunchKernelGGL((k_fused<T, 16>), g, dim3(64, 16), 0, s, P, C, ops); break;
+            }
         break;
     case 0: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 0>), g, bb, 0, s, P, C, ops); break;
     case 1: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 1>), g, bb, 0, s, P, C, ops); break;
@@ -876,11 +907,14 @@ void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, 
 }
 
 template <class T>
-void launch_fused_batch(const Scen<T> *sc, const Work *work, int nwork, int mask, hipStream_t s) {
+void launch_fused_batch(const Scen<T> *sc, const Work *work, int nwork, int mask, bool st, hipStream_t s) {
     if (nwork <= 0) return;
     const dim3 g(nwork), b8(64, 8), bb(64, BOUNDARY_WAVES);
     switch (mask) {
-    case -1: hipLaunchKernelGGL((k_fused_batch<T, 8>), g, b8, 0, s, sc, work); break;
+    case -1:
+        if (st) hipLaunchKernelGGL((k_fused_batch_stat<T, 8>), g, b8, 0, s, sc, work);
+        else hipLaunchKernelGGL((k_fused_batch<T, 8>), g, b8, 0, s, sc, work);
+        break;
     case 0: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 0>), g, bb, 0, s, sc, work); break;
     case 1: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 1>), g, bb, 0, s, sc, work); break;
     case 2: hipLaunchKernelGGL((k_fusedp_batch<T, BOUNDARY_WAVES, 2>), g, bb, 0, s, sc, work); break;
@@ -910,8 +944,8 @@ template void launch_zero_stats<double>(const StatZero<double> *, int, hipStream
 
 template FusedC<float> fused_consts<float>(const Params<float> &, const float *, const Work *);
 template FusedC<double> fused_consts<double>(const Params<double> &, const double *, const Work *);
-template void launch_fused_batch<float>(const Scen<float> *, const Work *, int, int, hipStream_t);
-template void launch_fused_batch<double>(const Scen<double> *, const Work *, int, int, hipStream_t);
+template void launch_fused_batch<float>(const Scen<float> *, const Work *, int, int, bool, hipStream_t);
+template void launch_fused_batch<double>(const Scen<double> *, const Work *, int, int, bool, hipStream_t);
 template void launch_fused<float>(const Params<float> &, const PointOps<float> &, const Work *, int, int,
                                   const float *, int, bool, hipStream_t);
 template void launch_fused<double>(const Params<double> &, const PointOps<double> &, const Work *, int, int,
