@@ -62,8 +62,12 @@ __device__ __forceinline__ void adj_modes(const T f[4], const T g[4], T h[4]) {
 }
 
 // raw node loads of one plane: ub 0-2, vb 3-5, u 6-8, v 9-11, v of the next
-// state 12-14, 1/m 15, gamma 16
-#define ADJ_NR 17
+// state 12-14, 1/m 15, gamma 16, the node's gradient sum 17.  The gradient
+// sums (node and cell) are loaded with the plane's other data, one plane
+// ahead, and stored with the step's term added: a load consumed at once would
+// make the wave wait for every load issued before it, the next plane's
+// prefetch included.
+#define ADJ_NR 18
 
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__restrict__ work) {
@@ -89,11 +93,13 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
             R[i] = A.uh[i][o], R[3 + i] = A.vh[i][o], R[6 + i] = A.u[i][o], R[9 + i] = A.v[i][o];
             R[12 + i] = A.vn[i][o];
         }
-        R[15] = A.mr[o], R[16] = A.gam[o];
+        R[15] = A.mr[o], R[16] = A.gam[o], R[17] = A.gm[o];
     };
-    auto fetch_cell = [&](int l, T (&M)[3]) {
+    // lambda, mu, yy and the three gradient sums of cell l
+    auto fetch_cell = [&](int l, T (&M)[6]) {
         const long o = g.at(jc, kc, min(l, lmax));
         M[0] = A.lam[o], M[1] = A.mu[o], M[2] = A.yy[o];
+        M[3] = A.gl[o], M[4] = A.gmu[o], M[5] = A.gy[o];
     };
     // exchanged: r = dt m (vb + dt ub) 0-2, w = u + gamma v 3-5, F = h1 u + dth2 v 6-8
     auto node_vals = [&](const T (&R)[ADJ_NR], T (&o9)[9]) {
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     };
     // cell (j, k, l): polynomials of Ke r (E: 1, s_j, s_k at s_l = +1; Ce at
     // s_l = -1) and of Kh r (H: s_j, s_k, s_j s_k; Ch), material sums if acc
-    auto cell_at = [&](int l, const T (&M)[3], const T (&Nlo)[4][9], const T (&Nhi)[4][9], T (&E)[3][3],
+    auto cell_at = [&](int l, const T (&M)[6], const T (&Nlo)[4][9], const T (&Nhi)[4][9], T (&E)[3][3],
                        T (&Ce)[3][3], T (&H)[3][3], T (&Ch)[3][3], bool acc) {
         const bool real = colreal && l >= A.lo[2] && l <= A.hi[2];
         T Gr[3][3], G[3][3];  // [a][c]: derivative along a of component c
@@ -181,9 +187,9 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int c = 0; c < 3; c++) sm = sm + Gr[a][c] * (G[a][c] + G[c][a]);
-            A.gl[o] += -V * tr * trr;
-            A.gmu[o] += -V * sm;
-            A.gy[o] += -sy;
+            A.gl[o] = M[3] + -V * tr * trr;
+            A.gmu[o] = M[4] + -V * sm;
+            A.gy[o] = M[5] + -sy;
         }
     };
 
@@ -191,14 +197,14 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     T NA[4][9], NB[4][9];
     T EA[3][3], HA[3][3];                              // polynomials of the current cell (s_l = +1)
     T CeA[3][3], ChA[3][3], CeB[3][3], ChB[3][3];      // s_l = -1 polynomials (carried, new in turn)
-    T RA[ADJ_NR], RB[ADJ_NR], mA[3], mB[3];
-    T ow[14];  // own node of the plane being updated: ub, vb, v, vn, 1/m, gamma
+    T RA[ADJ_NR], RB[ADJ_NR], mA[6], mB[6];
+    T ow[15];  // own node of the plane being updated: ub, vb, v, vn, 1/m, gamma, gradient sum
     auto keep_own = [&](const T (&R)[ADJ_NR]) {
 #pragma unroll
         for (int i = 0; i < 6; i++) ow[i] = R[i];
 #pragma unroll
         for (int i = 0; i < 3; i++) ow[6 + i] = R[9 + i], ow[9 + i] = R[12 + i];
-        ow[12] = R[15], ow[13] = R[16];
+        ow[12] = R[15], ow[13] = R[16], ow[14] = R[17];
     };
     fetch_node(wk.lz0 - 1, RA);
     node_vals(RA, o9);
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     fetch_cell(wk.lz0, mA);
     cell_at(wk.lz0 - 1, mB, NA, NB, EA, CeA, HA, ChA, false);
 
-    auto plane = [&](int l, bool act, const T (&Rc)[ADJ_NR], const T (&mc)[3], T (&Rn)[ADJ_NR], T (&mn)[3],
+    auto plane = [&](int l, bool act, const T (&Rc)[ADJ_NR], const T (&mc)[6], T (&Rn)[ADJ_NR], T (&mn)[6],
                      const T (&Nlo)[4][9], T (&Nhi)[4][9], const T (&cE)[3][3], const T (&cH)[3][3], T (&nE)[3][3],
                      T (&nH)[3][3]) {
         node_vals(Rc, o9);
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
                 A.vo[c][o] = vbp + ow[13] * fe[c] + dth2 * fh[c];
                 gm = gm + vbp * (ow[9 + c] - ow[6 + c]);
             }
-            if (ow[12] != T(0)) A.gm[o] += gm / ow[12];
+            if (ow[12] != T(0)) A.gm[o] = ow[14] + gm / ow[12];
         }
         keep_own(Rc);
     };

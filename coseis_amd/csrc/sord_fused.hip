@@ -65,6 +65,21 @@ __device__ __forceinline__ double lane_prev(double x) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// a global-memory pointer held in scalar registers: selecting between two
+// such pointers per lane stays a register select (a select between two
+// argument fields would become a per-lane load of the pointer, consumed at
+// once, which waits for every load in flight); typed as global memory so that
+// its accesses are global (not flat) loads and stores
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> uniform_ptr(T *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (gptr<T>)(T *)(((unsigned long long)hi << 32) | lo);
+}
+
 template <class T>
 __device__ __forceinline__ T wave_max(T v) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -471,6 +486,10 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     const int t = valid ? w * R * LW + lane : W * 64 + lane;  // LDS slot
     const int tab = valid && row + 1 < NR ? t + LW : t;     // row above
     const int tbe = valid && row > 0 ? t - LW : t;           // row below
+    // one damped axis PA: its PML memory comes with the plane's other loads
+    constexpr bool PF = MASK == 1 || MASK == 2 || MASK == 4;
+    constexpr int PA = MASK == 2 ? 1 : MASK == 4 ? 2 : 0;
+    constexpr int NMAT = PF ? 14 : 4;
     const int j = wk.jb - 1 + jj, k = wk.kb - 1 + row;
     const bool ownjk = valid && jj >= 1 && jj <= LW - 2 && row >= 1 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
     const bool cellv = valid && jj <= LW - 2 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
@@ -491,9 +510,83 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
         for (int i = 0; i < 3; i++) nd[i] = P.u[i][o], nd[3 + i] = P.v[i][o];
         nd[6] = P.gam[o];
     };
-    auto fetch_cell = [&](int l, T (&m)[4]) {
+    // PML memory of the damped axis id: element index, component stride,
+    // coefficient row and side of cell (j, k, l) (g, stress stage) or node
+    // (j, k, l) (p, accel stage); false where it has none
+    auto g_slot = [&](int id, int l, long &e, long &stride, int &pidx, bool &lo) -> bool {
+        const int c3[3] = {j, k, l};
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (c3[a] < P.i1cell[a] || c3[a] > P.i2cell[a]) return false;
+        if (fa >= 0 && c3[fa] == P.irup) return false;
+        const int c = c3[id];
+        lo = c <= P.i1pml[id];
+        if (!lo && c < P.i2pml[id] - 1) return false;
+        const int si = lo ? c - P.i1cell[id] + 1 : P.i2cell[id] - c + 1;
+        pidx = lo ? c + g.nnoff[id] : g.nn[id] - c - g.nnoff[id];
+        const int dep = lo ? P.gn[id][id] : P.gn[id + 3][id];
+        if (id == 0) {
+            e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
+            stride = (long)dep * g.nm[1] * g.nm[2];
+        } else if (id == 1) {
+            e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
+            stride = (long)g.nm[0] * dep * g.nm[2];
+        } else {
+            e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
+            stride = (long)g.nm[0] * g.nm[1] * dep;
+        }
+        return true;
+    };
+    auto p_slot = [&](int id, int l, long &e, long &stride, int &pidx, bool &lo) -> bool {
+        const int n3[3] = {j, k, l};
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (n3[a] < P.i1node[a] || n3[a] > P.i2node[a]) return false;
+        const int n = n3[id];
+        lo = n <= P.i1pml[id];
+        if (!lo && n < P.i2pml[id]) return false;
+        const int si = lo ? n - P.i1node[id] + 1 : P.i2node[id] - n + 1;
+        pidx = lo ? n + g.nnoff[id] : g.nn[id] - n - g.nnoff[id] + 1;
+        const int dep = lo ? P.pn[id][id] : P.pn[id + 3][id];
+        if (id == 0) {
+            e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
+            stride = (long)dep * g.nm[1] * g.nm[2];
+        } else if (id == 1) {
+            e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
+            stride = (long)g.nm[0] * dep * g.nm[2];
+        } else {
+            e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
+            stride = (long)g.nm[0] * g.nm[1] * dep;
+        }
+        return true;
+    };
+    // lambda, mu, yy of cell l and 1/m of node l; with one damped axis (PF)
+    // also the PML memory of cell l (g, 3 components, then dc2, dc1) and of
+    // node l (p, then dn2, dn1), loaded a plane ahead like the rest: a load
+    // consumed at once makes the wave wait for every load issued before it,
+    // the next plane's prefetch included
+    auto fetch_cell = [&](int l, T (&m)[NMAT]) {
         const long o = g.at(jc, kc, min(l, lmax));
         m[0] = P.lam[o], m[1] = P.mu[o], m[2] = P.yy[o], m[3] = P.mr[o];
+        if constexpr (PF) {
+            long e, stride;
+            int pidx;
+            bool lo;
+#pragma unroll
+            for (int q = 4; q < NMAT; q++) m[q] = T(0);
+            if (g_slot(PA, l, e, stride, pidx, lo)) {
+                const auto gp = lo ? uniform_ptr(P.gm[PA]) : uniform_ptr(P.gm[PA + 3]);
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) m[4 + ic] = gp[e + ic * stride];
+                m[7] = P.dc2[pidx - 1], m[8] = P.dc1[pidx - 1];
+            }
+            if (ownjk && p_slot(PA, l, e, stride, pidx, lo)) {
+                const auto pp = lo ? uniform_ptr(P.p[PA]) : uniform_ptr(P.p[PA + 3]);
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) m[9 + ic] = pp[e + ic * stride];
+                m[12] = P.dn2[pidx - 1], m[13] = P.dn1[pidx - 1];
+            }
+        }
     };
     // corner data of a plane: [corner dj + 2 dk][u 0..2, v 3..5, gamma 6]
     auto exchange = [&](const T (&own)[7], T (&N)[4][7]) {
@@ -520,7 +613,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     // swaps the two after the launch).  st: the cell counts in the statistics
     // (not the spare plane past the item's last, whose damping this kernel's
     // mask may lack).
-    auto cell_at = [&](int l, const T (&mat)[4], const T (&Nlo)[4][7], const T (&Nhi)[4][7], T (&E)[3][9],
+    auto cell_at = [&](int l, const T (&mat)[NMAT], const T (&Nlo)[4][7], const T (&Nhi)[4][7], T (&E)[3][9],
                        T (&Cc)[3][9], bool wr, bool st) {
         const int c3[3] = {j, k, l};
         bool real = true;
@@ -565,28 +658,22 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
             if ((MASK >> id & 1) && inS[id] && real) {
                 // split-field memory of the damped direction in the reference's
                 // units (volume-integrated strain), sord/stress.f90:42-131
-                const bool lo = inLo[id];
-                const int c = c3[id];
-                const int si = lo ? c - P.i1cell[id] + 1 : P.i2cell[id] - c + 1;
-                const int pidx = lo ? c + g.nnoff[id] : g.nn[id] - c - g.nnoff[id];
-                const T *gp = lo ? P.gm[id] : P.gm[id + 3];
-                T *gq = lo ? P.gmn[id] : P.gmn[id + 3];
-                const int dep = lo ? P.gn[id][id] : P.gn[id + 3][id];
                 long e, stride;
-                if (id == 0) {
-                    e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
-                    stride = (long)dep * g.nm[1] * g.nm[2];
-                } else if (id == 1) {
-                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
-                    stride = (long)g.nm[0] * dep * g.nm[2];
-                } else {
-                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
-                    stride = (long)g.nm[0] * g.nm[1] * dep;
-                }
-                const T c2 = P.dc2[pidx - 1], c1 = P.dc1[pidx - 1];
-                T old[3];
+                int pidx;
+                bool lo;
+                g_slot(id, l, e, stride, pidx, lo);  // true here: the cell is real and in the slab
+                const auto gp = lo ? uniform_ptr(P.gm[id]) : uniform_ptr(P.gm[id + 3]);
+                const auto gq = lo ? uniform_ptr(P.gmn[id]) : uniform_ptr(P.gmn[id + 3]);
+                T c2, c1, old[3];
+                if constexpr (PF) {
+                    c2 = mat[7], c1 = mat[8];
 #pragma unroll
-                for (int ic = 0; ic < 3; ic++) old[ic] = gp[e + ic * stride];
+                    for (int ic = 0; ic < 3; ic++) old[ic] = mat[4 + ic];
+                } else {
+                    c2 = P.dc2[pidx - 1], c1 = P.dc1[pidx - 1];
+#pragma unroll
+                    for (int ic = 0; ic < 3; ic++) old[ic] = gp[e + ic * stride];
+                }
 #pragma unroll
                 for (int ic = 0; ic < 3; ic++) {
                     const T dv = c2 * (G[id][ic] * C.vol) + c1 * old[ic];
@@ -666,7 +753,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     T own[7], E[3][9];
     T NA[4][7], NB[4][7];
     T CA[3][9], CB[3][9];
-    T fA[7], mA[4], fB[7], mB[4];
+    T fA[7], mA[NMAT], fB[7], mB[NMAT];
     fetch_node(wk.lz0 - 1, fA);
     exchange(fA, NA);
     fetch_node(wk.lz0, fB);
@@ -679,7 +766,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     fetch_cell(wk.lz0, mA);
     cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, false, true);
 
-    auto plane = [&](int l, bool act, const T (&cf)[7], const T (&cm)[4], T (&nf)[7], T (&nm_)[4], const T (&Nlo)[4][7],
+    auto plane = [&](int l, bool act, const T (&cf)[7], const T (&cm)[NMAT], T (&nf)[7], T (&nm_)[NMAT], const T (&Nlo)[4][7],
                      T (&Nhi)[4][7], const T (&carry)[3][9], T (&Cn)[3][9]) {
 #pragma unroll
         for (int q = 0; q < 7; q++) own[q] = cf[q];
@@ -735,28 +822,21 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
 #pragma unroll
             for (int id = 0; id < 3; id++) {
                 if (!(MASK >> id & 1)) continue;
-                const int n = n3[id];
-                const bool lo = n <= P.i1pml[id], hi = n >= P.i2pml[id];
-                if (!(lo || hi)) continue;
-                const int si = lo ? n - P.i1node[id] + 1 : P.i2node[id] - n + 1;
-                const int pidx = lo ? n + g.nnoff[id] : g.nn[id] - n - g.nnoff[id] + 1;
-                T *pp = lo ? P.p[id] : P.p[id + 3];
-                const int dep = lo ? P.pn[id][id] : P.pn[id + 3][id];
                 long e, stride;
-                if (id == 0) {
-                    e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
-                    stride = (long)dep * g.nm[1] * g.nm[2];
-                } else if (id == 1) {
-                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
-                    stride = (long)g.nm[0] * dep * g.nm[2];
-                } else {
-                    e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
-                    stride = (long)g.nm[0] * g.nm[1] * dep;
-                }
-                const T c2 = P.dn2[pidx - 1], c1 = P.dn1[pidx - 1];
-                T old[3];
+                int pidx;
+                bool lo;
+                if (!p_slot(id, l, e, stride, pidx, lo)) continue;  // not a PML node of axis id
+                const auto pp = lo ? uniform_ptr(P.p[id]) : uniform_ptr(P.p[id + 3]);
+                T c2, c1, old[3];
+                if constexpr (PF) {
+                    c2 = cm[12], c1 = cm[13];
 #pragma unroll
-                for (int ic = 0; ic < 3; ic++) old[ic] = pp[e + ic * stride];
+                    for (int ic = 0; ic < 3; ic++) old[ic] = cm[9 + ic];
+                } else {
+                    c2 = P.dn2[pidx - 1], c1 = P.dn1[pidx - 1];
+#pragma unroll
+                    for (int ic = 0; ic < 3; ic++) old[ic] = pp[e + ic * stride];
+                }
 #pragma unroll
                 for (int ic = 0; ic < 3; ic++) {
                     const T sv = c2 * D[ic][id] + c1 * old[ic];
