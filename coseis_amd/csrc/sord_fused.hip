@@ -210,6 +210,9 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
                                               const Work &wk) {
     __shared__ T sN[6][W][64];
     __shared__ T sC[6][W][64];
+    // statistics maxima of each lane (|a|^2, |v|^2, |u|^2, |w|^2), kept in LDS
+    // rather than in registers, which the plane loop has none to spare for
+    __shared__ T sS[ST ? 4 : 1][W][64];
     const int lane = threadIdx.x, w = threadIdx.y;
     const int j = wk.jb - 1 + lane, k = wk.kb - 1 + w;
     const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= wk.je && k <= wk.ke;
@@ -221,7 +224,12 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     const bool stat_now = ST && P.stat_now, stat_next = ST && P.stat_next;
     const unsigned tile_boxes = ops.n ? ops_boxes_jk(ops, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : 0u;
     const FsTile fst = FS ? fs_tile(P, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : FsTile{false, false};
-    T st_a = T(0), st_v = T(0), st_u = T(0), st_w = T(0);
+    if (ST)
+#pragma unroll
+        for (int q = 0; q < 4; q++) sS[ST ? q : 0][w][lane] = T(0);
+    auto st_max = [&](int q, T x) {
+        if (ST) sS[ST ? q : 0][w][lane] = tmax(sS[ST ? q : 0][w][lane], x);
+    };
 
     // node data: W = u + gamma v (strain), F = h1 u + dt h2 v (hourglass)
     T uo[3], vo[3];
@@ -302,7 +310,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
             cell_ops<T>(ops, bx, g.at(j, k, l), G, lm, m, C.vol, C.rvol, fq >= 0 ? P.fs_kind : 0, fv, s);
         }
         if (stat_now && cellv && st)
-            st_w = tmax(st_w, (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + T(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+            st_max(3, (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + T(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
         const T A[3][3] = {{C.ax * s[0], C.ay * s[5], C.az * s[4]},
                            {C.ax * s[5], C.ay * s[1], C.az * s[3]},
                            {C.ax * s[4], C.ay * s[3], C.az * s[2]}};
@@ -389,8 +397,8 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
                 v2 = v2 + vv * vv;
                 u2 = u2 + uu * uu;
             }
-            if (stat_now) st_a = tmax(st_a, a2);
-            if (stat_next) st_v = tmax(st_v, v2), st_u = tmax(st_u, u2);
+            if (stat_now) st_max(0, a2);
+            if (stat_next) st_max(1, v2), st_max(2, u2);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) uo[c] = cu[c], vo[c] = cv[c];
@@ -402,13 +410,13 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
         plane(l + 1, l + 1 <= wk.lz1, uB, vB, gB, mB, uA, vA, gA, mA, NA, NB, CB, CA);
     }
     if (stat_now) {
-        st_a = wave_max(st_a);
-        st_w = wave_max(st_w);
+        const T st_a = wave_max(sS[0][w][lane]);
+        const T st_w = wave_max(sS[ST ? 3 : 0][w][lane]);
         if (lane == 0) atomic_max_bits<T>(P.stat_cur + 0, st_a), atomic_max_bits<T>(P.stat_cur + 3, st_w);
     }
     if (stat_next) {
-        st_v = wave_max(st_v);
-        st_u = wave_max(st_u);
+        const T st_v = wave_max(sS[ST ? 1 : 0][w][lane]);
+        const T st_u = wave_max(sS[ST ? 2 : 0][w][lane]);
         if (lane == 0) atomic_max_bits<T>(P.stat_nxt + 1, st_v), atomic_max_bits<T>(P.stat_nxt + 2, st_u);
     }
 }
