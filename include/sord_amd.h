@@ -97,7 +97,7 @@ int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, in
 /* Select the execution path: 1 fused interior kernel where it applies
  * (default), 0 stage kernels everywhere. */
 int sord_set_fused(void *handle, int on);
-/* side = 0: the boundary kernels run in order on the handle's stream before
+/* side = 0: the boundary kernels run in order on the handle's stream after
  * the interior kernel, instead of on three side streams beside it (slower;
  * no cross-stream ordering).  Results are the same; default 1. */
 int sord_set_streams(void *handle, int side);
