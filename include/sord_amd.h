@@ -104,8 +104,9 @@ int sord_set_streams(void *handle, int side);
 
 /* Tile shape of the fused kernels: waves per block of the interior kernel
  * (4, 8 or 16; a block advances waves-2 node rows) and the planes one block
- * marches in the interior and boundary kernels.  Takes effect at the next
- * step; the results do not depend on it.  No counterpart in the reference
+ * marches in the interior and boundary kernels; the face slabs of the PML
+ * then get their own boundary launches at any mesh size.  Takes effect at
+ * the next step; the results do not depend on it.  No counterpart in the reference
  * (its loops have no tiles): the hook of the autotuner. */
 int sord_set_tiles(void *handle, int waves, int planes, int planes_boundary);
 

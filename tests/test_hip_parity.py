@@ -80,7 +80,11 @@ def test_results_do_not_depend_on_tiles(case):
     computation is inlined several times (tile prologue, the two planes of the
     unrolled loop, each wave count's kernel) and the compiler may contract
     other multiply-adds in each copy, so FP64 outputs agree to 1e-12 (1e-9
-    with a fault, whose friction law amplifies rounding)."""
+    with a fault, whose friction law amplifies rounding).  The base run puts
+    the whole boundary of these small meshes on the general kernel; with the
+    tiles set, the face slabs run on their own kernels (the one-axis
+    instantiations with the PML memory loaded a plane ahead), so the two are
+    checked against each other."""
     bar = 1e-9 if case == 'fault_sw' else 1e-12
     base = run_case(case, 'hip', 8)
     for tiles in ((8, 3, 5), (8, 40, 2), (16, 64, 16), (4, 64, 16)):
