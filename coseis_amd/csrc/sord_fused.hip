@@ -72,6 +72,8 @@ __device__ __forceinline__ double lane_prev(double x) {
 // its accesses are global (not flat) loads and stores
 template <class T>
 using gptr = __attribute__((address_space(1))) T *;
+// the same for an integer argument field (selected per lane between two)
+__device__ __forceinline__ int uniform_int(int x) { return __builtin_amdgcn_readfirstlane(x); }
 template <class T>
 __device__ __forceinline__ gptr<T> uniform_ptr(T *p) {
     const unsigned long long v = (unsigned long long)p;
@@ -532,7 +534,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
         if (!lo && c < P.i2pml[id] - 1) return false;
         const int si = lo ? c - P.i1cell[id] + 1 : P.i2cell[id] - c + 1;
         pidx = lo ? c + g.nnoff[id] : g.nn[id] - c - g.nnoff[id];
-        const int dep = lo ? P.gn[id][id] : P.gn[id + 3][id];
+        const int dep = lo ? uniform_int(P.gn[id][id]) : uniform_int(P.gn[id + 3][id]);
         if (id == 0) {
             e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
             stride = (long)dep * g.nm[1] * g.nm[2];
@@ -555,7 +557,7 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
         if (!lo && n < P.i2pml[id]) return false;
         const int si = lo ? n - P.i1node[id] + 1 : P.i2node[id] - n + 1;
         pidx = lo ? n + g.nnoff[id] : g.nn[id] - n - g.nnoff[id] + 1;
-        const int dep = lo ? P.pn[id][id] : P.pn[id + 3][id];
+        const int dep = lo ? uniform_int(P.pn[id][id]) : uniform_int(P.pn[id + 3][id]);
         if (id == 0) {
             e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
             stride = (long)dep * g.nm[1] * g.nm[2];
