@@ -18,13 +18,10 @@ roofline: the fused interior kernel (k_fused) is the dominant kernel; its
 algorithmic bytes per node-update are 68 B in FP32 (read u, v: 24 B, write u,
 v: 24 B, cell lambda, mu, yy: 12 B, node 1/m, gamma: 8 B); one launch advances
 the interior box, so achieved = 68 B x nodes / mean launch duration (HIP
-events on its stream; the boundary kernels run beside it on side streams, so
-the duration includes that sharing).  traffic: HBM bytes per launch from
-rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy with the
-same 4-byte access width), read from the committed summary in profiles/.
-roofline.alone: the same kernel over ALONE_STEPS further steps run with every
-kernel in order on one stream (sord_set_streams), so that its duration is its
-own, not shared with the boundary kernels.
+events around each launch on the solver stream, on which every kernel of the
+step runs in order).  traffic: HBM bytes per launch from rocprofv3
+FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy with the same
+4-byte access width), read from the committed summary in profiles/.
 step_roofline: the whole step, 68 B per node plus 48 B per node and damped
 axis (PML memory g and p), over the measured step time.
 
@@ -54,7 +51,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 ALG_BYTES = {4: 68, 8: 136}
-ALONE_STEPS = 50  # untimed for `value`: the dominant kernel measured on one stream after the timed steps
 WORKLOADS = {
     'layered': ('layered_321', 'layered half-space 321x321x161, dx 50 m, dt 0.004 s, free surface + '
                                '10-node PML, viscous hourglass, double couple, 16x16 receivers'),
@@ -94,7 +90,6 @@ def parse():
     p.add_argument('--tiles', type=int, nargs=3, default=None, metavar=('W', 'NZ', 'NZB'),
                    help='tile shape: interior waves, planes interior/boundary')
     p.add_argument('--no-batch', action='store_true', help='skip the batched-ensemble comparison')
-    p.add_argument('--no-alone', action='store_true', help='skip roofline.alone (the single-stream steps)')
     p.add_argument('--batch-scenarios', type=int, default=64)
     p.add_argument('--batch-steps', type=int, default=100)
     p.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient leg')
@@ -132,6 +127,16 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(prm, files, steps):
     """Time the reference solver (the amdflang build in oracle/_ref, when it was
     built) or else the oracle on `steps` steps of the same mesh, loop only."""
@@ -156,10 +161,14 @@ def cpu_baseline(prm, files, steps):
         sord.stage(prm)
         env = dict(os.environ, OMP_NUM_THREADS=str(cores))
         t0 = time.perf_counter()
-        subprocess.run([exe], check=True, capture_output=True, text=True, env=env)
+        r = subprocess.run([exe], check=True, capture_output=True, text=True, env=env)
         wall = time.perf_counter() - t0
-        # per-step loop time in ms (statistics.f90 prof-step.bin; the oracle writes it too)
-        ms = np.fromfile('prof-step.bin', '<i4').astype(float)
+        # per-step loop time in clock ticks (statistics.f90 prof-step.bin; the
+        # oracle writes it too) at the rate the solver prints first
+        # (sord/sord.f90:26, 'Clock rate')
+        m = re.search(r'^\s*(\d+)\s+Clock rate', r.stdout, re.M)
+        rate = float(m.group(1)) if m else 1000.0
+        ms = np.fromfile('prof-step.bin', '<i4').astype(float) * (1000.0 / rate)
     finally:
         os.chdir(cwd)
         shutil.rmtree(d)
@@ -170,6 +179,7 @@ def cpu_baseline(prm, files, steps):
         'unit': 'Gnode-updates/s',
         'cores': cores,
         'kind': kind,
+        'cpu': cpu_model(),
         'sample': '%d time steps of the same %dx%dx%d mesh and sources, FP32, OMP_NUM_THREADS=%d; '
                   'loop %.2f s of %.1f s wall' % (steps, prm['shape'][0], prm['shape'][1], prm['shape'][2],
                                                   cores, loop, wall),
@@ -289,8 +299,7 @@ def main():
         sys.exit(2)
     import torch
     from coseis_amd import configs, solver, sord
-    alone_steps = 0 if a.stage_only or a.no_alone else ALONE_STEPS
-    nt = a.warmup + a.steps + alone_steps
+    nt = a.warmup + a.steps
     prm, files = getattr(configs, WORKLOADS[a.workload][0])(nt=nt)
     nx, ny, nz = prm['shape'][:3]
     d = tempfile.mkdtemp(prefix='sord-bench-')
@@ -308,10 +317,6 @@ def main():
         ms_total, ms_kernel, knodes, kname = s.bench(a.steps)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        ms_alone = None
-        if alone_steps:  # the dominant kernel with the GPU to itself (after the timed steps)
-            s.set_streams(False)
-            ms_alone = s.bench(alone_steps)[1]
         s.close()
     finally:
         os.chdir(cwd)
@@ -331,7 +336,7 @@ def main():
         'config': {'workload': WORKLOADS[a.workload][1],
                    'nodes': nodes, 'realsize': a.realsize,
                    'path': 'stage kernels' if a.stage_only else
-                           'fused interior kernel + fused boundary kernels on side streams'},
+                           'fused interior kernel, then the fused boundary kernels, in order on one stream'},
         'wall_s': round(wall, 4),
         'roofline': {
             'bound': 'hbm', 'kernel': kname,
@@ -340,9 +345,6 @@ def main():
             'traffic': traffic_from_profile(),
             'alg_bytes_per_launch': alg, 'nodes_per_launch': knodes,
             'launch_ms': round(ms_kernel, 5),
-            'alone': None if not ms_alone else {
-                'launch_ms': round(ms_alone, 5), 'achieved': round(alg / (ms_alone / 1e3) / 1e9, 1),
-                'frac': round(alg / (ms_alone / 1e3) / 1e9 / HBM_PEAK_GBS, 4), 'steps': alone_steps},
         },
         'step_roofline': {
             'alg_bytes_per_step': step_bytes,

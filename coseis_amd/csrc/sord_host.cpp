@@ -258,7 +258,10 @@ struct Solver {
     T *red_val = nullptr;
     long *red_loc = nullptr;
     bool use_fused = true;
-    bool use_side = true;  // boundary launches on the side streams (else in order on the main stream)
+    // boundary launches on side streams beside the interior kernel (opt-in,
+    // sord_set_streams) or, by default, in order on the one solver stream:
+    // with side streams, outputs were seen to differ between processes
+    bool use_side = false;
     bool initialized = false;
     int state_bc_local = 1;  // all boundary conditions plane-local (-1, 1, 3, 0, pml)
 
@@ -949,6 +952,14 @@ extern "C" int sord_set_streams(void *h, int side) {
     return dispatch(h, [side](auto &S) { S.use_side = side != 0; });
 }
 
+extern "C" int sord_plan(void *h, int *nlaunch, int *masks, int cap) {
+    return dispatch(h, [&](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_plan before sord_init");
+        *nlaunch = S.fused_on ? (int)S.blaunch.size() : -1;
+        for (int i = 0; i < *nlaunch && i < cap; i++) masks[i] = S.blaunch[i].mask;
+    });
+}
+
 extern "C" int sord_set_tiles(void *h, int waves, int planes, int planes_boundary) {
     return dispatch(h, [&](auto &S) {
         if ((waves != 4 && waves != 8 && waves != 16) || planes < 1 || planes > 4096 || planes_boundary < 1 ||
@@ -1063,6 +1074,14 @@ extern "C" int sord_batch_set_tiles(void *batch, int index, int planes, int plan
             HIPCHECK(hipSetDevice(B.device));
             B.replan();
         }
+    });
+}
+
+extern "C" int sord_batch_set_streams(void *batch, int side) {
+    return dispatch_batch(batch, [&](auto &B) {
+        if (B.initialized) HIPCHECK(hipStreamSynchronize(B.stream));
+        B.drop_graphs();
+        B.use_side = side != 0;
     });
 }
 

@@ -18,8 +18,8 @@ ERRORS = {
 EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field', 'sord_set_tensor',
     'sord_tensor_numel',
-    'sord_bench', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_copy_probe',
-    'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
+    'sord_bench', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_plan', 'sord_copy_probe',
+    'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_streams', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
     'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
     'sord_adjoint_setup', 'sord_adjoint_forward', 'sord_adjoint_backward', 'sord_adjoint_timing',
     'sord_last_error', 'sord_destroy',
@@ -60,6 +60,7 @@ def load():
     lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
     lib.sord_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.sord_set_streams.argtypes = [vp, ctypes.c_int]
+    lib.sord_plan.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     lib.sord_copy_probe.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     lib.sord_last_error.restype = ctypes.c_char_p
     lib.sord_last_error.argtypes = []
@@ -68,6 +69,7 @@ def load():
     lib.sord_batch_create.argtypes = [ctypes.c_int, ccp, ccp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     lib.sord_batch_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.sord_batch_set_graph.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.sord_batch_set_streams.argtypes = [vp, ctypes.c_int]
     lib.sord_batch_init.argtypes = [vp]
     lib.sord_batch_step.argtypes = [vp, ctypes.c_int]
     lib.sord_batch_run.argtypes = [vp]
@@ -106,7 +108,7 @@ class Solver(object):
     output files, as the reference ``sord.x`` writes them into its cwd.
     """
 
-    def __init__(self, text, realsize=4, device=0, fused=True, tiles=None, side_streams=True):
+    def __init__(self, text, realsize=4, device=0, fused=True, tiles=None, side_streams=False):
         lib = load()
         self._h = ctypes.c_void_p()
         self._adopted = {}  # caller tensors the handle uses (set_tensor)
@@ -114,24 +116,42 @@ class Solver(object):
         self.realsize = realsize
         if not fused:
             _check(lib.sord_set_fused(self._h, 0))
-        if not side_streams:
-            _check(lib.sord_set_streams(self._h, 0))
+        if side_streams:
+            _check(lib.sord_set_streams(self._h, 1))
         if tiles:
             self.set_tiles(*tiles)
         _check(lib.sord_init(self._h))
 
     def set_streams(self, side=True):
-        """Boundary kernels on side streams (default) or in order on one stream."""
+        """Boundary kernels on side streams (opt-in) or in order on the
+        handle's one stream (the default)."""
         _check(_lib.sord_set_streams(self._h, 1 if side else 0))
 
     def set_tiles(self, waves=8, planes=16, planes_boundary=16):
         """Tile shape of the fused kernels (performance only; see sord_set_tiles)."""
         _check(_lib.sord_set_tiles(self._h, waves, planes, planes_boundary))
 
+    def _caller_sync(self):
+        # The library's stream is not ordered against torch's: work the caller
+        # queued on adopted tensors (fills, edits) must finish first.
+        if self._adopted:
+            import torch
+            torch.cuda.current_stream().synchronize()
+
+    def boundary_masks(self):
+        """Damped-axis masks of the planned boundary launches (sord_plan;
+        None when the stage kernels run the steps)."""
+        n = ctypes.c_int()
+        m = (ctypes.c_int * 64)()
+        _check(_lib.sord_plan(self._h, ctypes.byref(n), m, 64))
+        return None if n.value < 0 else [m[i] for i in range(min(n.value, 64))]
+
     def step(self, n=1):
+        self._caller_sync()
         _check(_lib.sord_step(self._h, n))
 
     def run(self):
+        self._caller_sync()
         _check(_lib.sord_run(self._h))
 
     def status(self):
@@ -209,6 +229,9 @@ class Solver(object):
         want = torch.float32 if self.realsize == 4 else torch.float64
         if tensor.dtype != want or not tensor.is_cuda or not tensor.is_contiguous():
             raise ValueError('set_tensor: a contiguous %s device tensor' % want)
+        # the adoption copies the handle's array into the tensor on the
+        # library's stream: the caller's pending work on it goes first
+        torch.cuda.current_stream().synchronize()
         _check(_lib.sord_set_tensor(self._h, name.encode(), ctypes.c_void_p(tensor.data_ptr()), tensor.numel()))
         self._adopted[name] = tensor
 
@@ -255,6 +278,11 @@ class Batch(object):
         u = ctypes.c_int()
         _check(_lib.sord_batch_set_graph(self._h, -1 if on is None else int(bool(on)), ctypes.byref(u)))
         return bool(u.value)
+
+    def set_streams(self, side=True):
+        """Merged boundary launches on side streams (opt-in) or in order on
+        the batch's one stream (the default)."""
+        _check(_lib.sord_batch_set_streams(self._h, 1 if side else 0))
 
     def set_tiles(self, index, planes, planes_boundary):
         """Planes per work item of one scenario; call replan() to apply."""

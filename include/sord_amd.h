@@ -97,9 +97,12 @@ int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, in
 /* Select the execution path: 1 fused interior kernel where it applies
  * (default), 0 stage kernels everywhere. */
 int sord_set_fused(void *handle, int on);
-/* side = 0: the boundary kernels run in order on the handle's stream after
- * the interior kernel, instead of on three side streams beside it (slower;
- * no cross-stream ordering).  Results are the same; default 1. */
+/* side = 1: the boundary kernels run on three side streams beside the
+ * interior kernel (forked and joined with events); side = 0 (the default):
+ * in order on the handle's stream after the interior kernel.  The kernels
+ * write disjoint nodes, but with side streams outputs were seen to differ
+ * between processes on one machine, so one stream is the default.  No
+ * counterpart in the reference (its loops run in order). */
 int sord_set_streams(void *handle, int side);
 
 /* Tile shape of the fused kernels: waves per block of the interior kernel
@@ -109,6 +112,12 @@ int sord_set_streams(void *handle, int side);
  * the next step; the results do not depend on it.  No counterpart in the reference
  * (its loops have no tiles): the hook of the autotuner. */
 int sord_set_tiles(void *handle, int waves, int planes, int planes_boundary);
+
+/* The current plan of the fused kernels: *nlaunch boundary launches per step
+ * (-1: the stage kernels run the steps) and, in masks[0..cap), the damped-axis
+ * specialisation of each (bit a: the kernel handles PML slabs of axis a; 7
+ * is the general kernel).  No counterpart in the reference: a check hook. */
+int sord_plan(void *handle, int *nlaunch, int *masks, int cap);
 
 /* Copy nbytes (a multiple of 4) between two device buffers of `device` with a
  * plain kernel that moves one 4-byte word per lane per access, and wait for
@@ -139,6 +148,11 @@ int sord_batch_init(void *batch);
  * -1: query only.  *in_use (if not NULL) receives 1 while graphs are in use (a
  * capture that fails switches them off).  Results are the same either way. */
 int sord_batch_set_graph(void *batch, int on, int *in_use);
+/* side = 1: the merged boundary launches on three side streams beside the
+ * interior launch; 0 (default): in order on the batch's stream, as
+ * sord_set_streams for one handle.  Drops the graphs (recaptured at the
+ * next step). */
+int sord_batch_set_streams(void *batch, int side);
 /* Advance every scenario that has steps left by up to nsteps steps. */
 int sord_batch_step(void *batch, int nsteps);
 int sord_batch_run(void *batch);

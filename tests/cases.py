@@ -316,3 +316,91 @@ def write_inputs(files, realsize=4):
     for name, a in files.items():
         a = np.asarray(a)
         a.astype('<i4' if np.issubdtype(a.dtype, np.integer) else dt).T.tofile(name)
+
+
+# ---------------------------------------------------------------------------
+# The benchmark workloads at their own mesh sizes (coseis_amd.configs), with
+# outputs sized for committed fixtures: receivers and planes decimated in
+# space and time.  FP64 reference runs of the two 16 M-node meshes take
+# about 2 s per step on 8 cores, so their FP64 windows are shorter than the
+# FP32 runs (which take the full step count).
+# ---------------------------------------------------------------------------
+ENSEMBLE_SUBSET = (41, 32, 63, 20)  # scenarios of configs.ensemble(): buckets 0, 1, 1, 2
+
+
+def w_explosion_61(realsize=4):
+    """Explosion, 61^3 nodes, 200 steps (the full run): the three velocity
+    receivers of the workload every step, a velocity plane through the source
+    and a decimated volume at the last step."""
+    from coseis_amd import configs
+    prm, files = configs.explosion_61(nt=200)
+    prm['vx'] = [prm['vx'][1], ([[], [], 30, -1], '=>', 'plane-vx.bin')]
+    prm['vy'] = [prm['vy'][1]]
+    prm['vz'] = [prm['vz'][1], ([[0, None, 3], [0, None, 3], [0, None, 3], -1], '=>', 'vol-vz.bin')]
+    return prm, files
+
+
+def w_layered_321(realsize=4):
+    """Layered half-space, 321 x 321 x 161 nodes: the 16 x 16 surface
+    receivers every 10th step (1500 steps in FP32, 300 in FP64) and the
+    surface vz every 4th node at the last step."""
+    from coseis_amd import configs
+    nt = 1500 if realsize == 4 else 300
+    prm, files = configs.layered_321(nt=nt)
+    for f in 'vx', 'vy', 'vz':
+        prm[f] = [([[8, 321, 20], [8, 321, 20], 0, [0, None, 10]], '=>', 'rcv-%s.bin' % f)]
+    prm['vz'].append(([[0, None, 4], [0, None, 4], 0, -1], '=>', 'snap-vz.bin'))
+    return prm, files
+
+
+def w_rupture_400(realsize=4):
+    """Spontaneous rupture, 400 x 200 x 200 nodes (1500 steps in FP32, 400 in
+    FP64, by which time the front has left the nucleation patch by about
+    6 km): slip, slip rate, shear traction and rupture time on every 4th x
+    and every 2nd z node of the fault plane at the last step, and velocity
+    at three off-fault receivers every step."""
+    from coseis_amd import configs
+    nt = 1500 if realsize == 4 else 400
+    prm, files = configs.rupture_400(nt=nt)
+    for f in ['sux', 'suy', 'suz', 'svm', 'tsm', 'trup']:
+        prm.pop(f, None)
+    sub = [[0, None, 4], [0, None, 2], -1]
+    for f in ['sux', 'svm', 'tsm', 'trup', 'tarr']:
+        prm[f] = [(sub, '=>', f + '.bin')]
+    for f in 'vx', 'vy', 'vz':
+        prm[f] = [([x, 104.0, 99.0, []], '.>', 'p%d-%s.bin' % (i, f)) for i, x in enumerate((199.5, 260.0, 330.0))]
+    return prm, files
+
+
+def w_adjoint_48(realsize=4):
+    """The adjoint workload's forward run on its 48^3 subgrid (the
+    generator's own material, sources and receivers; 1500 steps): velocity at
+    the 64 surface receivers every 5th step.  Material enters as '=<' cell
+    files, as coseis_amd.adjoint.Problem stages it."""
+    from coseis_amd import configs
+    w = configs.adjoint_200(n=48)
+    prm = dict(w['prm'])
+    files = {}
+    for name, a in zip(('rho', 'vp', 'vs'), w['model']):
+        files[name + '.bin'] = a
+        prm[name] = [([[], [], []], '=<', name + '.bin')]
+    for c in 'xyz':
+        prm['v' + c] = [([int(r[0]), int(r[1]), int(r[2]), [0, None, 5]], '=>', 'r%02d-v%s.bin' % (i, c))
+                        for i, r in enumerate(w['receivers'])]
+    return prm, files
+
+
+def w_ensemble(q, realsize=4):
+    """Scenario q of the batched ensemble (800 steps, top-plane vz at the end)."""
+    from coseis_amd import configs
+    return configs.ensemble()[q]
+
+
+WORKLOADS = {
+    'w_explosion_61': w_explosion_61,
+    'w_layered_321': w_layered_321,
+    'w_rupture_400': w_rupture_400,
+    'w_adjoint_48': w_adjoint_48,
+}
+for _q in ENSEMBLE_SUBSET:
+    WORKLOADS['w_ensemble_%02d' % _q] = (lambda q: lambda realsize=4: w_ensemble(q, realsize))(_q)

@@ -44,7 +44,7 @@ def collect(d, files, realsize):
     return out
 
 
-def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, side_streams=True, probe=None):
+def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, side_streams=False, probe=None):
     """Stage a case in a scratch directory and run it; returns {name: array}.
     probe(solver), if given, is called on the HIP solver before the run."""
     prm, files = cases.CASES[case]()

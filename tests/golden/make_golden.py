@@ -5,9 +5,12 @@ Runs the amdflang build of the reference SORD (oracle/_ref/r{4,8}/sord.x, made
 by oracle/build_ref.sh from /root/reference/sord) on every case in a scratch
 directory and stores every output file it writes as one array of an .npz
 archive per case and precision: tests/golden/<case>_r4.npz, <case>_r8.npz.
-Profile files (timings) are not stored.
+Profile files (timings) are not stored.  The cases are those of
+tests/cases.py: CASES (small meshes) and WORKLOADS (the benchmark workloads
+at their own sizes, whose definitions depend on the precision; the two
+16 M-node meshes take 10 to 50 minutes each on 8 cores).
 
-    python tests/golden/make_golden.py [case ...]
+    python tests/golden/make_golden.py [--r4 | --r8] [case ...]
 """
 import glob
 import os
@@ -26,8 +29,15 @@ from coseis_amd import sord  # noqa: E402
 import cases  # noqa: E402
 
 
+def case_inputs(case, realsize):
+    """(prm, files) of a case of tests/cases.py."""
+    if case in cases.CASES:
+        return cases.CASES[case]()
+    return cases.WORKLOADS[case](realsize)
+
+
 def run_reference(case, realsize, exe):
-    prm, files = cases.CASES[case]()
+    prm, files = case_inputs(case, realsize)
     d = tempfile.mkdtemp(prefix='golden-')
     cwd = os.getcwd()
     try:
@@ -48,11 +58,16 @@ def run_reference(case, realsize, exe):
 
 
 def main():
-    for rs in (4, 8):
+    args = sys.argv[1:]
+    sizes = [4, 8]
+    if '--r4' in args or '--r8' in args:
+        sizes = [rs for rs in sizes if '--r%d' % rs in args]
+    names = [a for a in args if not a.startswith('--')] or list(cases.CASES)
+    for rs in sizes:
         exe = os.path.join(ROOT, 'oracle', '_ref', 'r%d' % rs, 'sord.x')
         if not os.path.exists(exe):
             sys.exit('reference not built: run oracle/build_ref.sh')
-        for case in sys.argv[1:] or cases.CASES:
+        for case in names:
             out = run_reference(case, rs, exe)
             path = os.path.join(HERE, '%s_r%d.npz' % (case, rs))
             np.savez_compressed(path, **out)

@@ -95,10 +95,33 @@ def test_results_do_not_depend_on_tiles(case):
 
 @pytest.mark.parametrize('case', ['explosion_pml', 'layered_free'])
 def test_side_streams_do_not_change_results(case):
-    """The boundary kernels on side streams beside the interior kernel, or in
-    order on one stream: the same kernels compute every node, so the outputs
-    are identical (FP64)."""
-    a = run_case(case, 'hip', 8)
-    b = run_case(case, 'hip', 8, side_streams=False)
+    """The boundary kernels on side streams beside the interior kernel
+    (opt-in), or in order on one stream (the default): the same kernels
+    compute every node, so within one process the outputs are identical
+    (FP64)."""
+    a = run_case(case, 'hip', 8, side_streams=True)
+    b = run_case(case, 'hip', 8)
     for k in a:
         assert np.array_equal(a[k], b[k]), (k, rel_l2(a[k], b[k]))
+
+
+PML_CASES = ['explosion_pml', 'layered_free', 'sources_scattered', 'stages', 'strided_input']
+
+
+@pytest.mark.parametrize('realsize', [4, 8])
+@pytest.mark.parametrize('case', PML_CASES)
+def test_face_slab_kernels_match_reference(case, realsize):
+    """With the tile shape set, each PML face slab runs on its own boundary
+    kernel (the one-damped-axis instantiations, which load the PML memory a
+    plane ahead) instead of the general kernel that small meshes use by
+    default: every output against the reference at the bars of
+    test_hip_matches_reference."""
+    print('case', case, 'realsize', realsize, flush=True)
+    ref = golden(case, realsize)
+    names = []
+    out = run_case(case, 'hip', realsize, tiles=(8, 16, 4), probe=lambda s: names.append(s.boundary_masks()))
+    assert any(m in (1, 2, 4) for m in names[0]), names  # one-axis kernels ran
+    for k in ref:
+        bar = ORDER_FREE.get(k, TOL[realsize]) if realsize == 8 else TOL[realsize]
+        err = rel_l2(out[k], ref[k])
+        assert err <= bar, (k, err, bar)
