@@ -292,6 +292,166 @@ def topo_mesh():
     return prm, {'x.bin': x, 'y.bin': y, 'z.bin': z}
 
 
+def _fault_files(shp, **fields):
+    files = {}
+    for k, v in fields.items():
+        files['in-%s.bin' % k] = np.full(shp, v, 'f4') if np.isscalar(v) else np.asarray(v, 'f4')
+    return files
+
+
+def fault_prop():
+    """A rupture that propagates: a +y fault, slip-weakening with dc = 5 cm
+    on a 26 x 20 cell strength patch (mu_s = 1e4 around it), nucleation on
+    a 6 x 6 patch: the front crosses the patch and arrests at its edges, so
+    weakening to mu_d, front propagation, rupture-time interpolation and
+    arrest times are all exercised (slip reaches dc on 97 % of the patch)."""
+    nx, ny, nz = 36, 24, 30
+    prm = {
+        'shape': [nx, ny, nz, 100], 'delta': [100.0, 100.0, 100.0, 0.008],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 4,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0],
+        'faultnormal': '+y', 'hypocenter': [17.5, 11.0, 14.5],
+        'vrup': -1.0, 'itstats': 10, 'itio': 40,
+    }
+    shp = (nx, nz)
+    mus = np.full(shp, 1e4, 'f4')
+    mus[5:nx - 5, 5:nz - 5] = 0.677
+    ts = np.full(shp, 70e6, 'f4')
+    ts[15:21, 12:18] = 81.6e6
+    files = _fault_files(shp, mus=mus, mud=0.525, dc=0.05, tn=-120e6, ts=ts)
+    for f in ['mus', 'mud', 'dc', 'tn', 'ts']:
+        prm[f] = [([[], []], '=<', 'in-%s.bin' % f)]
+    for f in ['trup', 'tarr', 'sux', 'sl', 'psv', 'tsm', 'tnm']:
+        prm[f] = [([[], [], -1], '=>', f + '.bin')]
+    prm['svm'] = [([[], [], 40], '=>', 'svm-40.bin')]
+    prm['svx'] = [([24, 20, []], '=>', 'p-svx.bin')]
+    prm['vy'] = [([[], [], [], -1], '=>', 'vy.bin')]
+    return prm, files
+
+
+def kostrov():
+    """The Kostrov crack of the reference's test (cst/tests/sord_kostrov.py:
+    11-53): rupture forced to run at 0.9 vs from the hypocentre (rcrit 1e9,
+    trelax 0; mu_s = dc = 1e9, so friction never fails on its own), stress
+    drop 10 MPa on a +z fault under a cell-mirror face; slip rate every
+    step at four nodes 2 km from the nucleation centre on four azimuths.
+    The reference computes the nucleation centre by interpolating its
+    cell-centre array at the hypocentre's node index
+    (sord/dynamic_rupture.f90:176-203), so the hypocentre (39.5, 39.5)
+    puts the centre on node (52, 52), and the fault index 22 (the last cell
+    plane under the mirror) on the fault plane; a hypocentre on the mesh's
+    last node, as in the reference's quarter model, would read halo cells.
+    Fault parameters come from files (the reference's non-disk ops on fault
+    fields miss the plane).  The fault is locked in the PML (co = 1e20), 42
+    nodes from the centre: the stopping phases of the crack edge reach the
+    four points after the last step."""
+    nx, ny, nz = 105, 105, 24
+    prm = {
+        'shape': [nx, ny, nz, 200], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': 1.0, 'hourglass': [1.0, 1.0],
+        'bc1': ['pml', 'pml', 'pml'], 'bc2': ['pml', 'pml', '-cell'],
+        'faultnormal': '+z', 'hypocenter': [51.5, 51.5, 22.0],
+        'vrup': 0.9 * 3464.0, 'rcrit': 1e9, 'trelax': 0.0,
+    }
+    shp = (nx, ny)
+    files = _fault_files(shp, mud=1.0, mus=1e9, dc=1e9, tn=-90e6, ts=-90e6 - 10e6)
+    for f in ['mud', 'mus', 'dc', 'tn', 'ts']:
+        prm[f] = [([[], []], '=<', 'in-%s.bin' % f)]
+    prm['svm'] = [([x, y, []], '=>', 'p20%s.bin' % c) for c, (x, y) in zip('abcd', KOSTROV_POINTS)]
+    prm['trup'] = [([[], [], -1], '=>', 'trup.bin')]
+    return prm, files
+
+
+KOSTROV_CENTRE = (52, 52)
+KOSTROV_POINTS = ((72, 52), (64, 68), (68, 64), (52, 72))
+
+
+PULSES = ['const', 'delta', 'step', 'integral_delta', 'brune', 'integral_brune', 'hann', 'integral_hann',
+          'gaussian', 'integral_ricker1', 'ricker1', 'integral_ricker2', 'ricker2']
+
+
+def pulses():
+    """Every time function of sord/utilities.f90:223-282: the potency of
+    thirteen cells is set ('=') to val * f(t) each step, with the start
+    time tm0 before zero so that the pulses centred on t = 0 are sampled on
+    both sides, and read back ('=>') at the same cells every step."""
+    n = 16
+    prm = {
+        'shape': [n, n, n, 40], 'delta': [100.0, 100.0, 100.0, 0.0075], 'tm0': -0.06,
+        'bc1': ['free'] * 3, 'bc2': ['free'] * 3,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0], 'itstats': 5,
+        'pxx': [], 'vz': [([[], [], [], -1], '=>', 'vz.bin')],
+    }
+    for i, f in enumerate(PULSES):
+        c = [2 + (i % 4) * 3, 2 + (i // 4) * 3, 3 + (i % 3) * 4]
+        tau = {'delta': 0.0, 'step': 0.0, 'integral_delta': 0.0, 'const': 0.0}.get(f, 0.02)
+        prm['pxx'].append((c + [[]], '=', 1e-6 * (i + 1), f, tau))
+        prm['pxx'].append((c + [[]], '=>', 'pulse-%s.bin' % f))
+    return prm, {}
+
+
+def fault_opening():
+    """Rupture with fault opening (faultopening = 1: the normal traction
+    is capped at zero), prestress given as a stress tensor (sxx .. sxy)
+    plus strike and dip tractions (ts, td) and a normal traction that is
+    tensile on part of the fault, cohesion co, a slip vector off the axes,
+    on a -x fault."""
+    nx, ny, nz = 22, 24, 20
+    prm = {
+        'shape': [nx, ny, nz, 50], 'delta': [100.0, 100.0, 100.0, 0.008],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 4,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.02],
+        'faultnormal': '-x', 'hypocenter': [10.0, 11.5, 9.5], 'faultopening': 1,
+        'slipvector': [0.0, 0.8, 0.6], 'vrup': -1.0, 'itstats': 5,
+    }
+    shp = (ny, nz)
+    k, l = np.meshgrid(np.arange(ny), np.arange(nz), indexing='ij')
+    mus = np.full(shp, 1e4, 'f4')
+    mus[4:20, 4:16] = 0.6
+    tn = -40e6 + 60e6 * np.exp(-((k - 6.0) ** 2 + (l - 14.0) ** 2) / 8.0)  # tensile near (6, 14)
+    co = np.where((k > 14) & (l < 8), 2e6, 0.5e6)
+    files = _fault_files(shp, mus=mus, mud=0.45, dc=0.02, co=co, tn=tn, ts=30e6, td=-8e6,
+                         sxx=-20e6, syy=-25e6, szz=-30e6, syz=5e6, szx=-3e6, sxy=12e6)
+    files['in-ts.bin'][9:14, 7:12] = 45e6
+    for f in ['mus', 'mud', 'dc', 'co', 'tn', 'ts', 'td', 'sxx', 'syy', 'szz', 'syz', 'szx', 'sxy']:
+        prm[f] = [([[], []], '=<', 'in-%s.bin' % f)]
+    for f in ['trup', 'sux', 'suy', 'suz', 'tsm', 'sl']:
+        prm[f] = [([[], [], -1], '=>', f + '.bin')]
+    prm['nsx'] = [([[], []], '=>', 'nsx.bin')]
+    prm['tnm'] = [([[], [], -1], '=>', 'tnm.bin'), ([[], [], 24], '=>', 'tnm-25.bin')]
+    prm['vx'] = [([[], [], [], -1], '=>', 'vx.bin')]
+    return prm, files
+
+
+def material_ops():
+    """Material inputs through '*' and '*<' (multiply by a value, by a file)
+    and '+', the clamps rho/vp/vs/gam_min/max, and the velocity-dependent
+    viscosity vdamp (gam = vdamp / vs, sord/material_model.f90:35-50); the
+    clamped material and the moduli written at initialisation."""
+    nx, ny, nz = 18, 16, 14
+    rng = np.random.default_rng(31)
+    prm = {
+        'shape': [nx, ny, nz, 30], 'delta': [100.0, 100.0, 100.0, 0.0075],
+        'bc1': ['pml', 'pml', 'free'], 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2670.0, ([[], [], [0, 4]], '*', 0.95), ([[], [], [8, 13]], '*<', 'in-rho-scale.bin')],
+        'vp': [6000.0, ([[2, 10], [], []], '+', 150.0)],
+        'vs': [3464.0, ([[], [], []], '*<', 'in-vs-scale.bin')],
+        'gam': [0.0],
+        'rho_min': 2560.0, 'rho_max': 2700.0, 'vp_max': 6100.0, 'vs_min': 3300.0, 'vs_max': 3600.0,
+        'vdamp': 120.0, 'gam_min': 0.034, 'gam_max': 0.0355, 'hourglass': [1.0, 2.0], 'itstats': 5,
+        'mxy': [([9.0, 8.0, 6.0, []], '.', 1e16, 'brune', 0.04)],
+        'vx': [([[], [], [], -1], '=>', 'vx.bin')],
+        'vz': [([[], [], [], -1], '=>', 'vz.bin')],
+    }
+    for f in ['rho', 'vp', 'vs', 'gam', 'lam', 'mu', 'nu']:
+        prm[f] = list(prm.get(f, [])) + [([[], [], []], '=>', 'out-%s.bin' % f)]
+    files = {
+        'in-rho-scale.bin': rng.uniform(0.9, 1.1, (nx - 1, ny - 1, 5)),
+        'in-vs-scale.bin': rng.uniform(0.92, 1.06, (nx - 1, ny - 1, nz - 1)),
+    }
+    return prm, files
+
+
 CASES = {
     'explosion_pml': explosion_pml,
     'layered_free': layered_free,
@@ -307,6 +467,11 @@ CASES = {
     'affine_expand': affine_expand,
     'fault_affine': fault_affine,
     'topo_mesh': topo_mesh,
+    'fault_prop': fault_prop,
+    'kostrov': kostrov,
+    'pulses': pulses,
+    'fault_opening': fault_opening,
+    'material_ops': material_ops,
 }
 
 
@@ -329,44 +494,48 @@ ENSEMBLE_SUBSET = (41, 32, 63, 20)  # scenarios of configs.ensemble(): buckets 0
 
 
 def w_explosion_61(realsize=4):
-    """Explosion, 61^3 nodes, 200 steps (the full run): the three velocity
-    receivers of the workload every step, a velocity plane through the source
-    and a decimated volume at the last step."""
+    """Explosion, 61^3 nodes, 200 steps (the full run): velocity at the
+    workload's receiver on the x axis and at an off-axis receiver every
+    step, and every 3rd node of the volume at step 60 (the P front inside
+    the mesh)."""
     from coseis_amd import configs
     prm, files = configs.explosion_61(nt=200)
-    prm['vx'] = [prm['vx'][1], ([[], [], 30, -1], '=>', 'plane-vx.bin')]
-    prm['vy'] = [prm['vy'][1]]
-    prm['vz'] = [prm['vz'][1], ([[0, None, 3], [0, None, 3], [0, None, 3], -1], '=>', 'vol-vz.bin')]
+    for c in 'xyz':
+        prm['v' + c] = [prm['v' + c][1],
+                        ([44.0, 37.5, 25.0, []], '.>', 'p2-v%s.bin' % c),
+                        ([[0, None, 3], [0, None, 3], [0, None, 3], 59], '=>', 'vol-v%s.bin' % c)]
     return prm, files
 
 
 def w_layered_321(realsize=4):
     """Layered half-space, 321 x 321 x 161 nodes: the 16 x 16 surface
     receivers every 10th step (1500 steps in FP32, 300 in FP64) and the
-    surface vz every 4th node at the last step."""
+    surface velocity at every 4th node at step 250."""
     from coseis_amd import configs
     nt = 1500 if realsize == 4 else 300
     prm, files = configs.layered_321(nt=nt)
     for f in 'vx', 'vy', 'vz':
-        prm[f] = [([[8, 321, 20], [8, 321, 20], 0, [0, None, 10]], '=>', 'rcv-%s.bin' % f)]
-    prm['vz'].append(([[0, None, 4], [0, None, 4], 0, -1], '=>', 'snap-vz.bin'))
+        prm[f] = [([[8, 321, 20], [8, 321, 20], 0, [0, None, 10]], '=>', 'rcv-%s.bin' % f),
+                  ([[0, None, 4], [0, None, 4], 0, 249], '=>', 'snap-%s.bin' % f)]
     return prm, files
 
 
 def w_rupture_400(realsize=4):
     """Spontaneous rupture, 400 x 200 x 200 nodes (1500 steps in FP32, 400 in
     FP64, by which time the front has left the nucleation patch by about
-    6 km): slip, slip rate, shear traction and rupture time on every 4th x
-    and every 2nd z node of the fault plane at the last step, and velocity
-    at three off-fault receivers every step."""
+    6 km), on every 4th x and every 2nd z node of the fault plane: slip,
+    peak slip rate, shear traction, rupture and arrest time at the last
+    step, slip rate at step 250; velocity at three off-fault receivers every
+    step."""
     from coseis_amd import configs
     nt = 1500 if realsize == 4 else 400
     prm, files = configs.rupture_400(nt=nt)
     for f in ['sux', 'suy', 'suz', 'svm', 'tsm', 'trup']:
         prm.pop(f, None)
-    sub = [[0, None, 4], [0, None, 2], -1]
-    for f in ['sux', 'svm', 'tsm', 'trup', 'tarr']:
-        prm[f] = [(sub, '=>', f + '.bin')]
+    sub = [[0, None, 4], [0, None, 2]]
+    for f in ['sux', 'psv', 'tsm', 'trup', 'tarr']:
+        prm[f] = [(sub + [-1], '=>', f + '.bin')]
+    prm['svm'] = [(sub + [249], '=>', 'svm-250.bin')]
     for f in 'vx', 'vy', 'vz':
         prm[f] = [([x, 104.0, 99.0, []], '.>', 'p%d-%s.bin' % (i, f)) for i, x in enumerate((199.5, 260.0, 330.0))]
     return prm, files
@@ -391,9 +560,18 @@ def w_adjoint_48(realsize=4):
 
 
 def w_ensemble(q, realsize=4):
-    """Scenario q of the batched ensemble (800 steps, top-plane vz at the end)."""
+    """Scenario q of the batched ensemble (800 steps): besides the
+    workload's top-plane vz at the last step, the velocity of the top
+    interior plane at step 300 and at a receiver every step."""
     from coseis_amd import configs
-    return configs.ensemble()[q]
+    prm, files = configs.ensemble()[q]
+    nx, ny = prm['shape'][:2]
+    for c in 'xyz':
+        prm.setdefault('v' + c, [])
+        prm['v' + c] = list(prm['v' + c]) + [
+            ([[], [], 10, 299], '=>', 'top300-v%s.bin' % c),
+            ([nx // 2 + 0.5, ny // 3 + 0.25, 12.0, []], '.>', 'rcv-v%s.bin' % c)]
+    return prm, files
 
 
 WORKLOADS = {

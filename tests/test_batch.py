@@ -1,7 +1,8 @@
 """Batched ensembles: every scenario of a batch gives the output files of its
 own run (the batch kernels are other instantiations of the same kernel
 bodies, so FP64 agrees to rounding: 1e-12, 1e-9 with a fault, whose friction
-law amplifies rounding)."""
+law amplifies rounding), and each scenario's outputs match the reference's
+own run of that case (tests/golden, FP64 bars of check_outputs)."""
 import glob
 import os
 import shutil
@@ -10,7 +11,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import check_outputs, golden, rel_l2
 import cases
 from coseis_amd import sord
 
@@ -75,5 +76,9 @@ def test_batch_matches_separate_runs(graph):
             for k in a:
                 assert a[k].shape == o[k].shape, (c, k)
                 assert rel_l2(o[k], a[k]) < bar, (c, k, rel_l2(o[k], a[k]))
+            # and each scenario of the batch against the reference itself
+            ref = golden(c, realsize)
+            check_outputs(c, {k[:-4]: v for k, v in o.items() if k[:-4] in ref or k.startswith('stats-')},
+                          ref, ref, realsize)
     finally:
         shutil.rmtree(base)

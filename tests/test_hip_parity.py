@@ -2,50 +2,57 @@
 
 Tolerances: the HIP kernels use the rectilinear form of the support operators
 (differences first), exact box volumes and fused multiply-adds, so FP32 results
-differ from the reference's FP32 by rounding only; the bar is the target
-FP32 relative L2 error of 1e-4 on every output, and 1e-9 for FP64.
+differ from the reference's FP32 by rounding only.  Bars (relative L2; the x,
+y, z components of one vector output compared as one vector):
+
+* FP64: 1e-9 on every output (1e-6 on the fault energy sums, whose summation
+  order is free).
+* FP32: 1e-4 against the reference's FP32 run on every output that the
+  reference's own FP32 run reproduces to 1e-4 of its FP64 run.  An output the
+  reference's FP32 run misses by more is rounding noise in FP32 (the slip rate
+  of a rupture that has died out, fault_sw and fault_affine): there the HIP
+  FP32 run must be no farther from the FP64 reference than 1.5 times the
+  reference's FP32 run is, plus 1e-4.
+* Rupture and arrest times: where the reference's node ruptured (trup < 1e8)
+  or arrested: the same set of nodes, times within one time step (FP32) or
+  1e-9 s (FP64).
 """
 import numpy as np
 import pytest
 
-from conftest import golden, rel_l2, run_case
+from conftest import check_outputs, golden, rel_l2, run_case
 import cases
 
 pytestmark = pytest.mark.gpu
-TOL = {4: 1e-4, 8: 1e-9}
-# statistics sums whose order of summation is free (fault energies)
-ORDER_FREE = {'stats-efric': 1e-6, 'stats-estrain': 1e-6, 'stats-moment': 1e-6, 'stats-mw': 1e-6}
-
-
 @pytest.mark.parametrize('realsize', [4, 8])
 @pytest.mark.parametrize('case', sorted(cases.CASES))
 def test_hip_matches_reference(case, realsize):
-    """Every output file against the reference run in the same precision.
-
-    Rupture runs in FP32 are checked against the FP64 reference instead: the
-    reference's own FP32 run departs from its FP64 run by more than 1e-4 on
-    slip-rate outputs (threshold crossings of the friction law amplify
-    rounding), so the bar there is to be no farther from the FP64 oracle than
-    the reference's FP32 run is (plus 1e-4).
-    """
+    """Every output file against the reference run in the same precision."""
     print('case', case, 'realsize', realsize, flush=True)
     ref = golden(case, realsize)
     out = run_case(case, 'hip', realsize)
-    assert sorted(out) == sorted(ref)
-    rupture = cases.CASES[case]()[0].get('faultnormal', 'none') != 'none'
-    ref64 = golden(case, 8)
-    errs, bars = {}, {}
-    for k in ref:
-        assert out[k].shape == ref[k].shape, k
-        if realsize == 4 and rupture:
-            errs[k] = rel_l2(out[k], ref64[k])
-            bars[k] = max(TOL[4], 1.5 * rel_l2(ref[k], ref64[k]))
-        else:
-            errs[k] = rel_l2(out[k], ref[k])
-            bars[k] = ORDER_FREE.get(k, TOL[realsize]) if realsize == 8 else TOL[realsize]
-    print(case, realsize, {k: '%.2e/%.0e' % (errs[k], bars[k]) for k in errs}, flush=True)
-    bad = {k: (errs[k], bars[k]) for k in errs if not errs[k] <= bars[k]}
-    assert not bad, bad
+    check_outputs(case, out, ref, golden(case, 8), realsize)
+
+
+def test_kostrov_slip_rate_matches_closed_form():
+    """The HIP solver's Kostrov crack (FP32) against the closed form
+    (oracle/kostrov.py, restating cst/kostrov.py): slip rate at 2 km on four
+    azimuths at the last step within 3 % (the reference's own run of this
+    mesh is 1.4-2.0 % off, numerical dispersion), and rupture arrival at
+    each point within two time steps of r / vrup."""
+    import kostrov
+    prm = cases.CASES['kostrov']()[0]
+    nt, dt, vr = prm['shape'][3], prm['delta'][3], prm['vrup']
+    out = run_case('kostrov', 'hip', 4)
+    trup = out['trup'].reshape(prm['shape'][1], prm['shape'][0]).T
+    for p, (x, y) in zip('abcd', cases.KOSTROV_POINTS):
+        r = prm['delta'][0] * np.hypot(x - cases.KOSTROV_CENTRE[0], y - cases.KOSTROV_CENTRE[1])
+        v = kostrov.slip_rate(2670.0, 6000.0, 3464.0, vr, 10e6, r, (nt - 1.5) * dt - r / vr)
+        err = (out['p20' + p][-1] - v) / v
+        print('kostrov', p, 'slip rate %.4f closed form %.4f rel %.4f trup %.4f r/vrup %.4f' %
+              (out['p20' + p][-1], v, err, trup[x, y], r / vr), flush=True)
+        assert abs(err) < 0.03, (p, err)
+        assert abs(trup[x, y] - r / vr) < 2 * dt, (p, trup[x, y], r / vr)
 
 
 @pytest.mark.parametrize('case', ['explosion_pml', 'layered_free', 'fault_sw', 'finite_potency', 'finite_moment',
@@ -121,7 +128,4 @@ def test_face_slab_kernels_match_reference(case, realsize):
     names = []
     out = run_case(case, 'hip', realsize, tiles=(8, 16, 4), probe=lambda s: names.append(s.boundary_masks()))
     assert any(m in (1, 2, 4) for m in names[0]), names  # one-axis kernels ran
-    for k in ref:
-        bar = ORDER_FREE.get(k, TOL[realsize]) if realsize == 8 else TOL[realsize]
-        err = rel_l2(out[k], ref[k])
-        assert err <= bar, (k, err, bar)
+    check_outputs(case, out, ref, golden(case, 8), realsize)

@@ -34,3 +34,33 @@ def test_oracle_catches_planted_error():
     ref = golden('explosion_pml', 4)
     out = run_case('explosion_pml', 'oracle', 4, prm_update={'vpml': 3000.0})
     assert rel_l2(out['vx'], ref['vx']) > 1e-4
+
+
+def test_kostrov_constant_known_value():
+    """Dahlen's C for the reference's parameters (vrup = 0.9 vs,
+    vp = 6000, vs = 3464) is the 0.82 the reference's test names as its
+    alternative constant (cst/tests/sord_kostrov.py:58), within 0.1 %."""
+    import kostrov
+    c = kostrov.cee(0.9 * 3464.0 / 6000.0, 0.9)
+    assert abs(c / 0.82 - 1) < 1e-3, c
+
+
+def test_reference_kostrov_run_matches_closed_form():
+    """The reference's own run of the Kostrov case (the fixture) against the
+    closed form at the last step (cst/tests/sord_kostrov.py:54-76): slip rate
+    at 2 km on four azimuths, reduced time (nt - 1.5) dt - r / vrup.  The
+    reference states 1.5 % for its quarter model; on this full-plane mesh its
+    run sits 1.4-2.0 % above the closed form (numerical dispersion of the
+    100 m mesh), so the bar is 3 %; a wrong stress drop, rupture speed or
+    time origin moves the points by far more."""
+    import kostrov
+    prm = cases.CASES['kostrov']()[0]
+    nt, dt = prm['shape'][3], prm['delta'][3]
+    vr = prm['vrup']
+    for realsize in (4, 8):
+        ref = golden('kostrov', realsize)
+        for p, (x, y) in zip('abcd', cases.KOSTROV_POINTS):
+            r = prm['delta'][0] * np.hypot(x - cases.KOSTROV_CENTRE[0], y - cases.KOSTROV_CENTRE[1])
+            v = kostrov.slip_rate(2670.0, 6000.0, 3464.0, vr, 10e6, r, (nt - 1.5) * dt - r / vr)
+            err = (ref['p20' + p][-1] - v) / v
+            assert abs(err) < 0.03, (realsize, p, err)

@@ -1,0 +1,119 @@
+"""The benchmark workloads at their own mesh sizes against the reference.
+
+Fixtures (tests/golden/w_*_r{4,8}.npz) are the reference solver's own output
+files for the cases WORKLOADS of tests/cases.py: the explosion (61^3, 200
+steps), the layered half-space (321 x 321 x 161; 1500 steps in FP32, 300 in
+FP64), the spontaneous rupture (400 x 200 x 200; 1500 steps in FP32, 400 in
+FP64), the adjoint workload's forward run on its 48^3 subgrid (1500 steps)
+and four scenarios of the batched ensemble (800 steps, run here as one
+batch).  Bars (tests/conftest.py check_outputs): relative L2 of every output
+(vectors as a whole) <= 1e-4 in FP32 and <= 1e-9 in FP64; rupture time and
+arrest time where the reference's fault node ruptured (trup < 1e8): the same
+set of nodes, and times within one time step (FP32) or 1e-9 s (FP64).  An
+FP32 output that the reference's own FP32 run misses its FP64 run by more
+than 1e-4 (a field at the last step that has mostly left the mesh) is held
+to 1.5 times that distance plus 1e-4 from the FP64 reference.
+"""
+import os
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import check_outputs, collect, golden, rel_l2
+import cases
+from coseis_amd import sord
+
+pytestmark = pytest.mark.gpu
+TOL = {4: 1e-4, 8: 1e-9}
+
+
+def stage(name, realsize, d):
+    prm, files = cases.WORKLOADS[name](realsize)
+    cwd = os.getcwd()
+    try:
+        os.chdir(d)
+        cases.write_inputs(files, realsize)
+        text = sord.stage(prm, realsize=realsize)[0]
+    finally:
+        os.chdir(cwd)
+    return prm, files, text
+
+
+def run_workload(name, realsize):
+    from coseis_amd import solver
+    d = tempfile.mkdtemp(prefix='sord-%s-' % name)
+    cwd = os.getcwd()
+    try:
+        prm, files, text = stage(name, realsize, d)
+        os.chdir(d)
+        with solver.Solver(text, realsize=realsize) as s:
+            s.run()
+        return prm, collect(d, files, realsize)
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(d)
+
+
+def compare(name, prm, out, ref, realsize):
+    check_outputs(name, out, ref, golden(name, 8), realsize, dt=prm['delta'][3])
+
+
+@pytest.mark.parametrize('realsize', [4, 8])
+@pytest.mark.parametrize('name', ['w_explosion_61', 'w_layered_321', 'w_rupture_400'])
+def test_plain_workload_matches_reference(name, realsize):
+    print('workload', name, 'realsize', realsize, flush=True)
+    ref = golden(name, realsize)
+    prm, out = run_workload(name, realsize)
+    compare(name, prm, out, ref, realsize)
+
+
+@pytest.mark.parametrize('realsize', [4, 8])
+def test_adjoint_forward_matches_reference(realsize):
+    """The traces the gradient is taken of (coseis_amd.adjoint, checkpointed
+    forward run of the adjoint workload's 48^3 subgrid with its own
+    generator's material, sources and receivers) against the reference's
+    receiver outputs: traces[t] is the velocity at the start of step t + 1,
+    which is the reference's output of step t + 1."""
+    from coseis_amd import adjoint, configs
+    print('adjoint forward realsize', realsize, flush=True)
+    w = configs.adjoint_200(n=48)
+    prob = adjoint.Problem(w['prm'], w['receivers'], realsize=realsize)
+    tr = adjoint.traces(*w['model'], prob)
+    ref = golden('w_adjoint_48', realsize)
+    nrec = len(w['receivers'])
+    want = np.stack([np.stack([ref['r%02d-v%s' % (i, c)] for c in 'xyz'], -1) for i in range(nrec)], 1)
+    ref64 = golden('w_adjoint_48', 8)
+    want64 = np.stack([np.stack([ref64['r%02d-v%s' % (i, c)] for c in 'xyz'], -1) for i in range(nrec)], 1)
+    print('reference FP32 vs FP64 %.2e' % rel_l2(want, want64), flush=True)
+    got = tr[::5]
+    assert got.shape == want.shape, (got.shape, want.shape)
+    err = rel_l2(got, want)
+    print('adjoint forward traces rel L2 %.2e' % err, flush=True)
+    assert err <= TOL[realsize], err
+
+
+@pytest.mark.parametrize('realsize', [4, 8])
+def test_ensemble_subset_batch_matches_reference(realsize):
+    """Four scenarios of the batched ensemble (grid-size buckets 0, 1, 1, 2;
+    1 to 4 layers) run as one batch, each against its own reference run."""
+    from coseis_amd import solver
+    print('ensemble subset realsize', realsize, flush=True)
+    base = tempfile.mkdtemp(prefix='sord-ens-')
+    try:
+        names = ['w_ensemble_%02d' % q for q in cases.ENSEMBLE_SUBSET]
+        texts, dirs, meta = [], [], []
+        for n in names:
+            d = os.path.join(base, n)
+            os.makedirs(d)
+            prm, files, text = stage(n, realsize, d)
+            texts.append(text)
+            dirs.append(d)
+            meta.append((prm, files))
+        with solver.Batch(texts, dirs, realsize=realsize) as b:
+            b.run()
+        for n, d, (prm, files) in zip(names, dirs, meta):
+            compare(n, prm, collect(d, files, realsize), golden(n, realsize), realsize)
+    finally:
+        shutil.rmtree(base)
