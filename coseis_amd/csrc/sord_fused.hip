@@ -482,12 +482,17 @@ void k_fused_batch_stat(const Scen<T> *__restrict__ sc, const Work *__restrict__
 // Rows: an item of width je-jb+1 runs LW = width+2 lanes per node row (Work.pad0)
 // and R = 64/LW rows per wavefront, so narrow items (the x slabs) fill the
 // wavefront; node rows of the block are w*R + lane/LW, with halo rows 0 and W*R-1.
+// LDS of the boundary kernel: 7 node values and 21 cell coefficients per slot,
+// W * 64 + 64 slots (lanes past R*LW use the last 64)
+template <int W>
+constexpr int boundary_lds_slots() { return 28 * (W * 64 + 64); }
+
 template <class T, int W, int MASK, bool FS>
 __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
-                                              const Work &wk) {
-    constexpr int NS = W * 64 + 64;  // LDS slots per value; lanes past R*LW use the last 64
-    __shared__ T sN[7][NS];
-    __shared__ T sC[21][NS];
+                                              const Work &wk, T *lds) {
+    constexpr int NS = W * 64 + 64;
+    T (*sN)[NS] = reinterpret_cast<T (*)[NS]>(lds);
+    T (*sC)[NS] = reinterpret_cast<T (*)[NS]>(lds + 7 * NS);
     const int lane = threadIdx.x, w = threadIdx.y;
     const int LW = wk.pad0, R = 64 / LW, NR = W * R;
     const int jj = lane % LW, rr = lane / LW;
@@ -923,14 +928,16 @@ template <class T, int W, int MASK>
 __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;  // uniform over the block
-    boundary_item<T, W, MASK, false>(P, C, ops, wk);
+    __shared__ T lds[boundary_lds_slots<W>()];
+    boundary_item<T, W, MASK, false>(P, C, ops, wk, lds);
 }
 // a finite source: every boundary item on the kernel of all damped axes
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) void k_fusedp_fs(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;
-    boundary_item<T, W, 7, true>(P, C, ops, wk);
+    __shared__ T lds[boundary_lds_slots<W>()];
+    boundary_item<T, W, 7, true>(P, C, ops, wk, lds);
 }
 template <class T, int W, int MASK>
 __global__ __launch_bounds__(64 * W) void k_fusedp_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work) {
@@ -938,7 +945,8 @@ __global__ __launch_bounds__(64 * W) void k_fusedp_batch(const Scen<T> *__restri
     if (wk.lz0 > wk.lz1) return;
     const Scen<T> &S = sc[wk.pad1];
     if (!S.active) return;  // uniform over the block
-    boundary_item<T, W, MASK, false>(S.P, S.C, S.ops, wk);
+    __shared__ T lds[boundary_lds_slots<W>()];
+    boundary_item<T, W, MASK, false>(S.P, S.C, S.ops, wk, lds);
 }
 
 template <class T>

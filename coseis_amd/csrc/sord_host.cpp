@@ -949,7 +949,14 @@ extern "C" int sord_set_fused(void *h, int on) {
 }
 
 extern "C" int sord_set_streams(void *h, int side) {
-    return dispatch(h, [side](auto &S) { S.use_side = side != 0; });
+    return dispatch(h, [side](auto &S) {
+        S.use_side = side != 0;
+        if (S.initialized) {  // the boundary launches are planned per stream mode
+            HIPCHECK(hipSetDevice(S.device));
+            HIPCHECK(hipStreamSynchronize(S.stream));
+            S.plan_regions();
+        }
+    });
 }
 
 extern "C" int sord_plan(void *h, int *nlaunch, int *masks, int cap) {
@@ -1082,6 +1089,10 @@ extern "C" int sord_batch_set_streams(void *batch, int side) {
         if (B.initialized) HIPCHECK(hipStreamSynchronize(B.stream));
         B.drop_graphs();
         B.use_side = side != 0;
+        if (B.initialized) {
+            HIPCHECK(hipSetDevice(B.device));
+            B.merge_lists();
+        }
     });
 }
 

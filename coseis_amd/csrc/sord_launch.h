@@ -26,7 +26,7 @@ template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, h
 
 namespace sord {
 // mask -1: interior kernel with `waves` waves per block; 0..7: boundary kernel
-// specialised on the damped axes (bit a: axis a), 8 waves per block
+// specialised on the damped axes (bit a: axis a)
 template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, bool fs, hipStream_t s);
 // constants of the fused kernels for one scenario; the batch launch takes a
 // device table of scenarios, each work item naming its own (Work.pad1)

@@ -38,8 +38,9 @@ def test_kostrov_slip_rate_matches_closed_form():
     """The HIP solver's Kostrov crack (FP32) against the closed form
     (oracle/kostrov.py, restating cst/kostrov.py): slip rate at 2 km on four
     azimuths at the last step within 3 % (the reference's own run of this
-    mesh is 1.4-2.0 % off, numerical dispersion), and rupture arrival at
-    each point within two time steps of r / vrup."""
+    mesh is 1.4-2.0 % off, numerical dispersion), and the rupture time
+    (slip rate crossing svtol) at each point within four time steps of
+    r / vrup (the reference's own run: 2 to 3 steps after it)."""
     import kostrov
     prm = cases.CASES['kostrov']()[0]
     nt, dt, vr = prm['shape'][3], prm['delta'][3], prm['vrup']
@@ -52,7 +53,7 @@ def test_kostrov_slip_rate_matches_closed_form():
         print('kostrov', p, 'slip rate %.4f closed form %.4f rel %.4f trup %.4f r/vrup %.4f' %
               (out['p20' + p][-1], v, err, trup[x, y], r / vr), flush=True)
         assert abs(err) < 0.03, (p, err)
-        assert abs(trup[x, y] - r / vr) < 2 * dt, (p, trup[x, y], r / vr)
+        assert abs(trup[x, y] - r / vr) < 4 * dt, (p, trup[x, y], r / vr)
 
 
 @pytest.mark.parametrize('case', ['explosion_pml', 'layered_free', 'fault_sw', 'finite_potency', 'finite_moment',
