@@ -133,6 +133,14 @@ struct Params {
     int stat_now, stat_next;
 };
 
+// Partial maxima of the max-norm reduction: at most this many blocks.
+constexpr int MAXNORM_BLOCKS = 1024;
+// Layout of the statically allocated scratch of the allocation-free step
+// (sord_step_static): the two sets of four statistics slots, then the
+// reduction's partial maxima and their locations (8 bytes each in FP64).
+constexpr size_t SCRATCH_STAT = 0, SCRATCH_VAL = 256, SCRATCH_LOC = SCRATCH_VAL + 8 * MAXNORM_BLOCKS,
+                 SCRATCH_BYTES = SCRATCH_LOC + 8 * MAXNORM_BLOCKS;
+
 // One block of the fused kernels: node columns [jb, je] x [kb, ke] (at most
 // 62 x (W-2)) over planes [lz0, lz1]; lz0 > lz1 marks padding.
 // waves per block of the boundary kernel, and the widest column segment of its

@@ -155,6 +155,7 @@ struct IoRec {
     int ib = -1;
     std::vector<T> hbuf;   // host buffer (nb records)
     T *dbuf = nullptr;     // device staging (nb records) for device outputs
+    size_t dcap = 0;       // its capacity (values)
     size_t rec = 0;
 };
 
@@ -281,7 +282,12 @@ struct Solver {
         if (stream && own_streams) (void)hipStreamDestroy(stream);
     }
 
+    // set while sord_step_static runs: any device allocation is an error
+    bool no_alloc = false;
+    unsigned char *sscratch = nullptr;  // this device's static scratch (static_scratch)
+
     T *dalloc(size_t n) {
+        if (no_alloc) fail(SORD_ESTATE, "sord_step_static: this step would allocate device memory");
         void *p = nullptr;
         HIPCHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
         HIPCHECK(hipMemsetAsync(p, 0, (n ? n : 1) * sizeof(T), stream));
@@ -884,6 +890,14 @@ extern "C" int sord_step(void *h, int nsteps) {
         HIPCHECK(hipSetDevice(S.device));
         for (int i = 0; i < nsteps && S.it < S.nt; i++) S.step_one();
         HIPCHECK(hipStreamSynchronize(S.stream));
+    });
+}
+
+extern "C" int sord_step_static(void *h, int nsteps) {
+    return dispatch(h, [nsteps](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_step_static before sord_init");
+        HIPCHECK(hipSetDevice(S.device));
+        S.step_static(nsteps);
     });
 }
 

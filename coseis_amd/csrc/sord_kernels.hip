@@ -17,6 +17,8 @@
 //   diff_cn(s, a)(node) = sum over the 8 cells of (w_b w_c / 4) * (+-s)
 // with w_b, w_c the cell widths across a (the exact value of the reference's
 // general hexahedral formula, sord/diff_nc_op.f90:22-67, for a box).
+#include <mutex>
+
 #include "sord_dev.h"
 #include "sord_launch.h"
 
@@ -1030,7 +1032,7 @@ int launch_maxnorm(T *const a[3], T *const b[3], const Grid<T> &g, const int lo[
     for (int i = 0; i < 3; i++) bx.lo[i] = lo[i], bx.hi[i] = hi[i], bx.st[i] = 1, bx.n[i] = hi[i] - lo[i] + 1;
     const long N = (long)bx.n[0] * bx.n[1] * bx.n[2];
     int nb = nblocks(N);
-    if (nb > 1024) nb = 1024;
+    if (nb > MAXNORM_BLOCKS) nb = MAXNORM_BLOCKS;
     hipLaunchKernelGGL((k_maxnorm<T>), dim3(nb), dim3(256), 0, s, a[0], a[1], a[2], b ? b[0] : nullptr,
                        b ? b[1] : nullptr, b ? b[2] : nullptr, g, bx, pval, ploc);
     return nb;
@@ -1107,6 +1109,32 @@ void launch_copy_words(const void *src, void *dst, long nwords, hipStream_t s) {
     const long blocks = std::min<long>((nwords + 255) / 256, 65536);
     hipLaunchKernelGGL(k_copy_words, dim3((unsigned)blocks), dim3(256), 0, s, (const unsigned *)src, (unsigned *)dst,
                        nwords);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free step (sord_step_static): the statistics
+// slots and the partial maxima of the max-norm reduction, in a file-scope
+// device array of SCRATCH_BYTES that every device holds from the library's
+// load until the process ends.  Host code never passes the array's name to a
+// kernel or a copy (on the host it is a stand-in, not a device address): it
+// uses the address hipGetSymbolAddress gives, taken once per device with that
+// device current, at the handle's initialisation.
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+void *static_scratch(int device) {
+    static std::mutex mu;
+    static void *addr[64] = {};
+    if (device < 0 || device >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        addr[device] = p;
+    }
+    return addr[device];
 }
 
 }  // namespace sord

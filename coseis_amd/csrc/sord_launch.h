@@ -16,6 +16,9 @@ template <class T> void launch_sample(const T *f, View g, const Samples<T> &smp,
 template <class T> void launch_point_add(T *f, View g, const int lo[3], const int hi[3], const T xi[3], T val, hipStream_t s);
 template <class T> void launch_norm(T *out, T *const a[3], T *const b[3], const Grid<T> &g, const int lo[3], const int hi[3], const int st[3], const int n[3], hipStream_t s);
 void launch_copy_words(const void *src, void *dst, long nwords, hipStream_t s);
+// device address of the static scratch (SCRATCH_BYTES) on `device`, which must
+// be current; nullptr if it cannot be taken
+void *static_scratch(int device);
 template <class T> int launch_maxnorm(T *const a[3], T *const b[3], const Grid<T> &g, const int lo[3], const int hi[3], T *pval, long *ploc, hipStream_t s);
 template <class T> void launch_vbc_plane(const Params<T> &P, int ax, int dst, int src, int mask, const T sg[3], int kind, hipStream_t s);
 template <class T> void launch_rupture(const Params<T> &P, const Fault<T> &F, int it, T tm, int fused, hipStream_t s);

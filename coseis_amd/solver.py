@@ -16,7 +16,7 @@ ERRORS = {
     7: 'EBADOP', 8: 'EFILE', 9: 'EHIP', 10: 'EUNSUPPORTED', 11: 'ESTATE',
 }
 EXPORTS = [
-    'sord_create', 'sord_init', 'sord_step', 'sord_run', 'sord_status', 'sord_field', 'sord_set_tensor',
+    'sord_create', 'sord_init', 'sord_step', 'sord_step_static', 'sord_run', 'sord_status', 'sord_field', 'sord_set_tensor',
     'sord_tensor_numel',
     'sord_bench', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_plan', 'sord_copy_probe',
     'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_streams', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
@@ -49,6 +49,7 @@ def load():
     lib.sord_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     lib.sord_init.argtypes = [vp]
     lib.sord_step.argtypes = [vp, ctypes.c_int]
+    lib.sord_step_static.argtypes = [vp, ctypes.c_int]
     lib.sord_run.argtypes = [vp]
     lib.sord_status.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     i64p = ctypes.POINTER(ctypes.c_int64)
@@ -149,6 +150,11 @@ class Solver(object):
     def step(self, n=1):
         self._caller_sync()
         _check(_lib.sord_step(self._h, n))
+
+    def step_static(self, n=1):
+        """As step(), with no device allocation (sord_step_static)."""
+        self._caller_sync()
+        _check(_lib.sord_step_static(self._h, n))
 
     def run(self):
         self._caller_sync()

@@ -53,6 +53,19 @@ int sord_step(void *handle, int nsteps);
 /* Run all remaining steps and flush every output file (sord/sord.f90:39-49). */
 int sord_run(void *handle);
 
+/* sord_step without device allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory while stepping (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * step's scratch (the two sets of statistics slots and the partial maxima of
+ * the max-norm reduction, at most 1024 blocks) lives in a file-scope device
+ * array of SCRATCH_BYTES (16.6 KB) that each device holds until the process
+ * ends; output buffers are allocated by sord_init.  Same results as
+ * sord_step, and the two may alternate on one handle.  Calls on one device
+ * share that array: calls that overlap in time on one device (two threads,
+ * two handles) race.  A step that would allocate (none does after sord_init)
+ * fails with SORD_ESTATE; SORD_EHIP if the array's address cannot be taken. */
+int sord_step_static(void *handle, int nsteps);
+
 /* Current step (0 after init) and total steps. */
 int sord_status(void *handle, int *it, int *nt);
 
