@@ -207,219 +207,243 @@ time_functions = [
     'ricker2',
 ]
 
-_nonfile_ops = ['.', '=', '+', '*', '=~', '+~', '*~']
-_cube_ops = ['=@', '+@', '*@']
-_file_ops = ['.<', '=<', '+<', '*<', '.>', '=>']
+# field-I/O operation kinds: value ops (optionally times a time function),
+# cube ops (a value inside a coordinate box), file ops (read '<', write '>')
+VALUE_OPS = ('.', '=', '+', '*', '=~', '+~', '*~')
+CUBE_OPS = ('=@', '+@', '*@')
+FILE_OPS = ('.<', '=<', '+<', '*<', '.>', '=>')
+AXES = {'x': 0, 'y': 1, 'z': 2}
 
 
 class SordError(Exception):
     """A staging or solver error (bad parameters, bad i/o op, solver abort)."""
 
 
-def expand_slices(shape, slices):
-    """Normalise user indices to ``[start, stop, step]`` lists.
+def _axis_range(n, s):
+    """One axis of a user index as [start, stop, step] over n points: a list
+    is Python slice notation ([] is the whole axis, README.md:166-170), a
+    number is a single (possibly negative) index."""
+    if not isinstance(s, (list, tuple)):
+        return [s % n, s % n + 1, 1]
+    if len(s) == 0:
+        return [0, n, 1]
+    return list(slice(*s).indices(n))
 
-    Follows cst/sord.py:226-239 with the ``[]`` shorthand meaning the whole
-    axis (README.md:166-170) and lists instead of tuples.
-    """
-    n = len(shape)
+
+def expand_slices(shape, slices):
+    """User indices of a field (one entry per axis of `shape`, or none for the
+    whole field) as [start, stop, step] lists (cst/sord.py:226-239)."""
     if len(slices) == 0:
-        slices = n * [[]]
-    elif len(slices) != n:
+        slices = [[]] * len(shape)
+    if len(slices) != len(shape):
         raise SordError('error in indices: %r' % (slices,))
-    ss = []
-    for n, s in zip(shape, slices):
-        if isinstance(s, (list, tuple)):
-            s = list(slice(*s).indices(n)) if len(s) else [0, n, 1]
+    return [_axis_range(n, s) for n, s in zip(shape, slices)]
+
+
+def _count(r):
+    """Number of indices in a [start, stop, step] range."""
+    start, stop, step = r
+    return (stop - start - 1) // step + 1
+
+
+def _fault_plane(prm, nn):
+    """Fault-normal axis and the index of the fault's first node plane (None,
+    None without a fault).  The hypocentre wraps like a negative index; a
+    fault on the first or last cell plane of its axis makes that face a
+    cell-centred mirror (cst/sord.py:306-317)."""
+    if prm['faultnormal'] == 'none':
+        return None, None
+    ax = AXES[prm['faultnormal'][1]]
+    hyp = prm['hypocenter']
+    for i in range(3):
+        hyp[i] %= nn[i]
+        if hyp[i] > nn[i] - 1:
+            raise SordError('hypocenter outside the mesh')
+    irup = int(hyp[ax])
+    if irup == 0:
+        prm['bc1'][ax] = '-cell'
+    if irup == nn[ax] - 2:
+        prm['bc2'][ax] = '-cell'
+    return ax, irup
+
+
+def _pml_box(prm, nn):
+    """Last damped node on the low side and first on the high side, per
+    axis, as the solver reads them (it adds 1): exactly npml damped node and
+    cell planes per PML face, none on other faces (-1 and n + 1).  The
+    shipped stager's box (cst/sord.py:319-330) damps npml + 1 planes against
+    npml table entries (defect D13)."""
+    npml = prm['npml']
+    lo = [-1, -1, -1]
+    hi = [n + 1 for n in nn]
+    if npml <= 0:
+        return lo, hi
+    for i in range(3):
+        if prm['bc1'][i] == 'pml':
+            lo[i] = npml - 1
+        if prm['bc2'][i] == 'pml':
+            hi[i] = nn[i] - npml + 1
+        if lo[i] + 1 >= hi[i] - 1:
+            raise SordError('Error: model too small for PML')
+    return lo, hi
+
+
+def _op_list(value):
+    """The operations given for one field: a number, one operation tuple, or
+    a list of numbers and tuples."""
+    if isinstance(value, (float, int)):
+        return [value]
+    if len(value) > 1 and isinstance(value[1], str):
+        return [value]
+    return list(value)
+
+
+class _Op(object):
+    """One field-I/O operation decoded from its tuple."""
+
+    def __init__(self, item, files):
+        self.fname, self.val, self.tau = 'const', 0.0, 0.0
+        self.x1, self.x2 = [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]
+        if isinstance(item, (float, int)):
+            self.index, self.op, args = [], '=', [item]
         else:
-            s = [s % n, s % n + 1, 1]
-        ss.append(s)
-    return ss
+            self.index, self.op, args = item[0], item[1], list(item[2:])
+        op = self.op
+        if op == '#':
+            pass
+        elif op in VALUE_OPS:
+            if len(args) == 1:
+                self.val = args[0]
+            else:
+                self.val, self.fname, self.tau = args
+        elif op in CUBE_OPS:
+            if len(args) == 3:
+                x1, x2, self.val = args
+            else:
+                x1, x2, self.val, self.fname, self.tau = args
+            self.x1, self.x2 = list(x1), list(x2)
+        elif op in FILE_OPS:
+            self.fname, = args
+            if self.fname in files:
+                raise SordError('Error: duplicate filename: ' + self.fname)
+            files.add(self.fname)
+            self.fname = os.path.expanduser(self.fname)
+        else:
+            raise SordError('Error: bad i/o operation: %r' % (item,))
+        if len(self.fname) > 32:
+            raise SordError('File/function name too long: ' + self.fname)
+        if op not in FILE_OPS and op != '#' and self.fname not in time_functions:
+            raise SordError('invalid time func: %r' % (self.fname,))
+
+
+class _Stager(object):
+    """The solver's field-I/O records and the output metadata of one run."""
+
+    def __init__(self, prm, fault_axis, irup):
+        self.prm = prm
+        self.nn = prm['shape'][:3]
+        self.nt = prm['shape'][3]
+        self.fault_axis, self.irup = fault_axis, irup
+        self.files = set()
+        self.meta = {'indices': {}, 'deltas': {}, 'shapes': {}}
+
+    def extent(self, reg, static):
+        """Index extent of a field: nodes or cells, without the fault-normal
+        axis on the fault plane, plus time for time-varying fields."""
+        ext = [n - 1 for n in self.nn] if reg == 'c' else list(self.nn)
+        if reg == 'f':
+            del ext[self.fault_axis]
+        return ext if static else ext + [self.nt]
+
+    def buffer_depth(self, ranges):
+        """Time records buffered per file write (performance only): up to
+        itio steps, 1 for a slice larger than (nx + ny + nz)^2 points, at
+        most itbuff for other multi-point slices."""
+        nx, ny, nz = self.nn
+        nb = max(1, min((min(self.prm['itio'], self.nt) - 1) // ranges[3][2] + 1, _count(ranges[3])))
+        points = _count(ranges[0]) * _count(ranges[1]) * _count(ranges[2])
+        if points > (nx + ny + nz) ** 2:
+            return 1
+        return min(nb, self.prm['itbuff']) if points > 1 else nb
+
+    def describe(self, o, ranges):
+        """meta.json entries of a file op: the file's shape (axes of more than
+        one point), their spacing and the indices it covers."""
+        index, shape, delta = [], [], []
+        for i, (start, stop, step) in enumerate(ranges):
+            n = _count([start, stop, step])
+            if n == 1:
+                index.append(start)
+                continue
+            shape.append(n)
+            delta.append(self.prm['delta'][i] * step)
+            index.append([start, stop] if step == 1 else [start, stop, step])
+        if shape:
+            self.meta['shapes'][o.fname] = shape
+        if delta:
+            self.meta['deltas'][o.fname] = delta
+        self.meta['indices'][o.fname] = o.x1 if '.' in o.op else index
+
+    def field(self, name, value):
+        """Records of one field and their time window (first, last step)."""
+        if name not in fieldnames:
+            raise SordError('unknown field: %r' % name)
+        tags = fieldnames[name][0]
+        reg, settable, static = tags[0], '<' in tags, '~' not in tags
+        if reg == 'f' and self.fault_axis is None:
+            raise SordError('Error: field only for ruptures: %r' % name)
+        ext = self.extent(reg, static)
+        recs, first, last = [], self.nt + 1, -1
+        for item in _op_list(value):
+            o = _Op(item, self.files)
+            if '>' not in o.op and not settable:
+                raise SordError('Error: field is ouput only: %r' % name)
+            ranges = expand_slices(ext, o.index)
+            if reg == 'f':
+                ranges.insert(self.fault_axis, [self.irup, self.irup + 1, 1])
+            if static:
+                ranges.append([-1, 0, 1])  # applied at initialisation (it = 0)
+            if '.' in o.op:
+                # a sub-cell point: x1 keeps the fractional node (or cell)
+                # position, the range the cell that holds it
+                top = [n - 1 for n in self.nn] if reg == 'c' else list(self.nn)
+                for i in range(3):
+                    o.x1[i] = float(ranges[i][0])
+                    if o.x1[i] > top[i] - 1:
+                        raise SordError('sub-cell location outside the mesh')
+                    ranges[i][0], ranges[i][1] = int(ranges[i][0]), int(ranges[i][1])
+            first, last = min(first, ranges[3][0]), max(last, ranges[3][1])
+            recs.append([name, reg, ranges, self.buffer_depth(ranges), o.x1, o.x2, o.val, o.tau, o.op, o.fname])
+            if o.op[-1] in '<>':
+                self.describe(o, ranges)
+        return first, last, recs
 
 
 def prepare(prm, fio):
-    """Derive the solver input records; cst/sord.py:297-468."""
+    """Solver parameters (line 1 of sord.in), field-I/O records (one line
+    each) and output metadata from the user's parameters and field I/O;
+    the semantics of cst/sord.py:297-468 with defects D3-D5 and D13 repaired.
+
+    Records are grouped per field and the groups ordered by time window
+    (first step, last step, then the records themselves), which is the order
+    the solver scans them in (sord/input_output.f90:103-151)."""
     prm = copy.deepcopy(prm)
     if prm['source'] not in ('potency', 'moment', 'force', 'none'):
         raise SordError('bad source type: %r' % prm['source'])
     if prm['faultnormal'] not in ('none', '-x', '-y', '-z', '+x', '+y', '+z'):
         raise SordError('bad faultnormal: %r' % prm['faultnormal'])
-
-    # intervals
-    nx, ny, nz, nt = shape = prm['shape']
+    nn, nt = prm['shape'][:3], prm['shape'][3]
     prm['itio'] = max(1, min(prm['itio'], nt))
-
-    # rupture
-    ifn = None
-    irup = None
-    if prm['faultnormal'] != 'none':
-        ifn = {'x': 0, 'y': 1, 'z': 2}[prm['faultnormal'][1:]]
-        for i in 0, 1, 2:
-            prm['hypocenter'][i] %= shape[i]
-            if prm['hypocenter'][i] > shape[i] - 1:
-                raise SordError('hypocenter outside the mesh')
-        irup = int(prm['hypocenter'][ifn])
-        if irup == 0:
-            prm['bc1'][ifn] = '-cell'
-        if irup == shape[ifn] - 2:
-            prm['bc2'][ifn] = '-cell'
-
-    # pml (D13: npml damped planes per PML face, none on other faces)
-    npml = prm['npml']
-    i1 = [-1, -1, -1]
-    i2 = [nx + 1, ny + 1, nz + 1]
-    if npml > 0:
-        for i in 0, 1, 2:
-            if prm['bc1'][i] == 'pml':
-                i1[i] = npml - 1
-            if prm['bc2'][i] == 'pml':
-                i2[i] = shape[i] - npml + 1
-            if i1[i] + 1 >= i2[i] - 1:
-                raise SordError('Error: model too small for PML')
-    prm.update({'i1pml': i1, 'i2pml': i2})
-
-    # convert boundary conditions to numeric ids
-    for k in 'bc1', 'bc2':
-        for i in 0, 1, 2:
-            if isinstance(prm[k][i], str):
-                prm[k][i] = boundary_conditions[prm[k][i]]
-
-    # field i/o
-    fio_ = []
-    filenames = []
-    shapes = {}
-    deltas = {}
-    indices = {}
-    for field in sorted(fio):
-        if field not in fieldnames:
-            raise SordError('unknown field: %r' % field)
-        ios = fio[field]
-        tags = fieldnames[field][0]
-        reg = tags[0]
-        input_ = '<' in tags
-        static = '~' not in tags
-        it1, it2 = nt + 1, -1
-        if reg == 'c':
-            fshape = [nx - 1, ny - 1, nz - 1]
-        else:
-            fshape = [nx, ny, nz]
-        if reg == 'f':
-            if ifn is None:
-                raise SordError('Error: field only for ruptures: %r' % field)
-            del fshape[ifn]
-        if not static:
-            fshape += [nt]
-        if isinstance(ios, (float, int)):
-            ios = [ios]
-        elif len(ios) > 1 and isinstance(ios[1], str):
-            ios = [ios]
-        ios_ = []
-        for io in ios:
-            fname, val, tau = 'const', 0.0, 0.0
-            x1, x2 = [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]
-            if isinstance(io, (float, int)):
-                slices, op, rhs = [], '=', [io]
-            else:
-                slices, op, rhs = io[0], io[1], list(io[2:])
-            if op == '#':
-                pass
-            elif op in _nonfile_ops:
-                if len(rhs) == 1:
-                    val, = rhs
-                else:
-                    val, fname, tau = rhs
-            elif op in _cube_ops:
-                if len(rhs) == 3:
-                    x1, x2, val = rhs
-                else:
-                    x1, x2, val, fname, tau = rhs
-                x1 = list(x1)
-                x2 = list(x2)
-            elif op in _file_ops:
-                fname, = rhs
-                if fname in filenames:
-                    raise SordError('Error: duplicate filename: ' + fname)
-                filenames.append(fname)
-                fname = os.path.expanduser(fname)
-            else:
-                raise SordError('Error: bad i/o operation: %r' % (io,))
-
-            # error check
-            if len(fname) > 32:
-                raise SordError('File/function name too long: ' + fname)
-            if op not in _file_ops and op != '#' and fname not in time_functions:
-                raise SordError('invalid time func: %r' % fname)
-            if '>' not in op and not input_:
-                raise SordError('Error: field is ouput only: %r' % field)
-
-            # slices
-            slices = expand_slices(fshape, slices)
-            if reg == 'f':
-                slices.insert(ifn, [irup, irup + 1, 1])
-            if static:
-                slices.append([-1, 0, 1])
-            if '.' in op:
-                full = [nx - 1, ny - 1, nz - 1] if reg == 'c' else [nx, ny, nz]
-                for i in 0, 1, 2:
-                    x1[i] = float(slices[i][0])
-                    if x1[i] > full[i] - 1:
-                        raise SordError('sub-cell location outside the mesh')
-                    slices[i][0] = int(slices[i][0])
-                    slices[i][1] = int(slices[i][1])
-            it1 = min(it1, slices[-1][0])
-            it2 = max(it2, slices[-1][1])
-
-            # buffer size
-            nn = [(s[1] - s[0] - 1) // s[2] + 1 for s in slices]
-            nb = (min(prm['itio'], nt) - 1) // slices[3][2] + 1
-            nb = max(1, min(nb, nn[3]))
-            n = nn[0] * nn[1] * nn[2]
-            if n > (nx + ny + nz) ** 2:
-                nb = 1
-            elif n > 1:
-                nb = min(nb, prm['itbuff'])
-
-            ios_ += [[field, reg, slices, nb, x1, x2, val, tau, op, fname]]
-
-            # metadata
-            if op[-1] in '<>':
-                ii = []
-                nn = []
-                dd = []
-                for i, s in enumerate(slices):
-                    start, stop, step = s
-                    d = prm['delta'][i] * step
-                    n = (stop - start - 1) // step + 1
-                    if n == 1:
-                        ii += [start]
-                    else:
-                        dd += [d]
-                        nn += [n]
-                        if step == 1:
-                            ii += [[start, stop]]
-                        else:
-                            ii += [[start, stop, step]]
-                if nn != []:
-                    shapes[fname] = nn
-                if dd != []:
-                    deltas[fname] = dd
-                if '.' in op:
-                    indices[fname] = x1
-                else:
-                    indices[fname] = ii
-
-        fio_ += [(it1, it2, ios_)]
-
-    # sort by time window (cst/sord.py:460)
-    fio_ = [g for f in sorted(fio_) for g in f[2]]
+    ax, irup = _fault_plane(prm, nn)
+    prm['i1pml'], prm['i2pml'] = _pml_box(prm, nn)
+    for key in ('bc1', 'bc2'):
+        prm[key] = [boundary_conditions[b] if isinstance(b, str) else b for b in prm[key]]
+    st = _Stager(prm, ax, irup)
+    groups = sorted(st.field(name, fio[name]) for name in sorted(fio))
+    records = [r for g in groups for r in g[2]]
     del prm['itbuff']
-    prm.update({'nfieldio': len(fio_)})
-    meta = {
-        'indices': indices,
-        'deltas': deltas,
-        'shapes': shapes,
-    }
-    return prm, fio_, meta
+    prm['nfieldio'] = len(records)
+    return prm, records, st.meta
 
 
 def split_args(args=None, **kwargs):
