@@ -207,7 +207,10 @@ __device__ __forceinline__ void hg_modes(T f00, T f10, T f01, T f11, T g00, T g1
 // ---------------------------------------------------------------------------
 // interior tiles
 // ---------------------------------------------------------------------------
-template <class T, int W, bool FS, bool ST>
+// DEPTH: planes of node data in flight ahead of the plane being computed (1:
+// two register sets alternate; 2: three rotate, for 8-wave blocks that run
+// one block per CU, whose waves have the register file of two)
+template <class T, int W, bool FS, bool ST, int DEPTH = 1>
 __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
                                               const Work &wk) {
     __shared__ T sN[6][W][64];
@@ -334,24 +337,33 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
         }
     };
 
-    // The plane loop is unrolled by two with alternating register sets (A/B):
-    // the loads for plane l+2 go to the set that plane l+1 does not read, so
-    // no register holding a load in flight is ever copied (a copy would make
-    // the compiler wait for the load at the end of every iteration).
+    // The plane loop is unrolled with rotating register sets: the loads for
+    // plane l+1+DEPTH go to the set that no plane before them reads, so no
+    // register holding a load in flight is ever copied (a copy would make the
+    // compiler wait for the load at the end of every iteration).
     T own6[6], E[3][4];
     T NA[4][6], NB[4][6];  // corner data of two planes (lower, upper in turn)
     T CA[3][4], CB[3][4];  // s_l = -1 polynomials of two cells (carried, new in turn)
     T uA[3], vA[3], gA, mA[4], uB[3], vB[3], gB, mB[4];  // loads: plane l+1 raw, cell l + 1/m
+    T uC[3], vC[3], gC, mC[4];                            // the third set (DEPTH 2)
     fetch_node(wk.lz0 - 1, uA, vA, gA);
     node_vals(uA, vA, gA, own6);
     exchange(own6, NA);
     fetch_node(wk.lz0, uo, vo, gB);
     fetch_cell(wk.lz0 - 1, mB);
+    if constexpr (DEPTH == 2) {
+        fetch_node(wk.lz0 + 1, uC, vC, gC);
+        fetch_cell(wk.lz0, mC);
+        fetch_node(wk.lz0 + 2, uA, vA, gA);
+        fetch_cell(wk.lz0 + 1, mA);
+    }
     node_vals(uo, vo, gB, own6);
     __syncthreads();  // every wave has read the first plane from LDS
     exchange(own6, NB);
-    fetch_node(wk.lz0 + 1, uA, vA, gA);
-    fetch_cell(wk.lz0, mA);
+    if constexpr (DEPTH == 1) {
+        fetch_node(wk.lz0 + 1, uA, vA, gA);
+        fetch_cell(wk.lz0, mA);
+    }
     cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, true);
 
     // one plane: cu..cm hold plane l+1 and cell l (arrived), nu..nm receive plane
@@ -360,8 +372,8 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     auto plane = [&](int l, bool act, T (&cu)[3], T (&cv)[3], T &cg, T (&cm)[4], T (&nu)[3], T (&nv)[3], T &ng,
                      T (&nm)[4], T (&Nlo)[4][6], T (&Nhi)[4][6], T (&carry)[3][4], T (&Cn)[3][4]) {
         node_vals(cu, cv, cg, own6);
-        fetch_node(l + 2, nu, nv, ng);
-        fetch_cell(l + 1, nm);
+        fetch_node(l + 1 + DEPTH, nu, nv, ng);
+        fetch_cell(l + DEPTH, nm);
         exchange(own6, Nhi);
         cell_at(l, cm, Nlo, Nhi, E, Cn, act);
         T R[3][2];
@@ -405,11 +417,24 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
 #pragma unroll
         for (int c = 0; c < 3; c++) uo[c] = cu[c], vo[c] = cv[c];
     };
-    // both planes of a pair always run (one loop back-edge keeps the register
-    // sets in place); past the last plane of the item the second one stores nothing
-    for (int l = wk.lz0; l <= wk.lz1; l += 2) {
-        plane(l, true, uA, vA, gA, mA, uB, vB, gB, mB, NB, NA, CA, CB);
-        plane(l + 1, l + 1 <= wk.lz1, uB, vB, gB, mB, uA, vA, gA, mA, NA, NB, CB, CA);
+    // all planes of a group always run (one loop back-edge keeps the register
+    // sets in place); past the last plane of the item they store nothing
+    if constexpr (DEPTH == 1) {
+        for (int l = wk.lz0; l <= wk.lz1; l += 2) {
+            plane(l, true, uA, vA, gA, mA, uB, vB, gB, mB, NB, NA, CA, CB);
+            plane(l + 1, l + 1 <= wk.lz1, uB, vB, gB, mB, uA, vA, gA, mA, NA, NB, CB, CA);
+        }
+    } else {
+        // three load sets (C, A, B hold planes l+1, l+2, l+3 in turn) and the
+        // two corner and polynomial sets: six planes per back-edge
+        for (int l = wk.lz0; l <= wk.lz1; l += 6) {
+            plane(l, true, uC, vC, gC, mC, uB, vB, gB, mB, NB, NA, CA, CB);
+            plane(l + 1, l + 1 <= wk.lz1, uA, vA, gA, mA, uC, vC, gC, mC, NA, NB, CB, CA);
+            plane(l + 2, l + 2 <= wk.lz1, uB, vB, gB, mB, uA, vA, gA, mA, NB, NA, CA, CB);
+            plane(l + 3, l + 3 <= wk.lz1, uC, vC, gC, mC, uB, vB, gB, mB, NA, NB, CB, CA);
+            plane(l + 4, l + 4 <= wk.lz1, uA, vA, gA, mA, uC, vC, gC, mC, NB, NA, CA, CB);
+            plane(l + 5, l + 5 <= wk.lz1, uB, vB, gB, mB, uA, vA, gA, mA, NA, NB, CB, CA);
+        }
     }
     if (stat_now) {
         const T st_a = wave_max(sS[0][w][lane]);
@@ -430,20 +455,31 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
 // VGPRs, and the spill reloads in the plane loop waited for every load in
 // flight, the next plane's prefetch included (C2: 0.482 against 0.427 ms per
 // interior launch).
+#ifndef SORD_INTERIOR_DEPTH
+#define SORD_INTERIOR_DEPTH 1
+#endif
+// prefetch depth and waves per SIMD of the plain interior kernels: FP32
+// 8-wave blocks (one per CU) may take the deeper prefetch and the register
+// file of two waves per SIMD; 16-wave blocks need four
 template <class T, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+constexpr int interior_depth() { return sizeof(T) == 4 && W == 8 ? SORD_INTERIOR_DEPTH : 1; }
+template <class T, int W>
+constexpr int interior_waves_per_eu() { return sizeof(T) == 4 ? (interior_depth<T, W>() == 2 ? 2 : 4) : 1; }
+
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(interior_waves_per_eu<T, W>())))
 void k_fused(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;  // uniform over the block
-    interior_item<T, W, false, false>(P, C, ops, wk);
+    interior_item<T, W, false, false, interior_depth<T, W>()>(P, C, ops, wk);
 }
 // the same on a statistics step
 template <class T, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(interior_waves_per_eu<T, W>())))
 void k_fused_stat(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;
-    interior_item<T, W, false, true>(P, C, ops, wk);
+    interior_item<T, W, false, true, interior_depth<T, W>()>(P, C, ops, wk);
 }
 // with a finite source: its box enters the stress or the force
 template <class T, int W>
@@ -948,6 +984,21 @@ __global__ __launch_bounds__(64 * W) void k_fusedp_batch(const Scen<T> *__restri
     __shared__ T lds[boundary_lds_slots<W>()];
     boundary_item<T, W, MASK, false>(S.P, S.C, S.ops, wk, lds);
 }
+
+// blocks of the general boundary kernel (the most registers of the boundary
+// kernels) one CU holds at once
+template <class T>
+int boundary_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fusedp<T, BOUNDARY_WAVES, 7>, 64 * BOUNDARY_WAVES, 0) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
+}
+template int boundary_blocks_per_cu<float>();
+template int boundary_blocks_per_cu<double>();
 
 template <class T>
 FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work) {

@@ -33,6 +33,7 @@ namespace sord {
 template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, bool fs, hipStream_t s);
 // constants of the fused kernels for one scenario; the batch launch takes a
 // device table of scenarios, each work item naming its own (Work.pad1)
+template <class T> int boundary_blocks_per_cu();
 template <class T> FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work);
 template <class T> void launch_adjoint(const AdjP<T> &A, const Work *work, int nwork, hipStream_t s);
 template <class T> void launch_adj_inject(T *const vh[3], const Grid<T> &g, const int *rec, int n, const T *res, hipStream_t s);
