@@ -704,8 +704,27 @@ struct Solver {
                                     else if (op[0] == '+') f(j, k, l) = f(j, k, l) + val;
                                     else f(j, k, l) = f(j, k, l) * val;
                                 }
+                    } else if (op[1] == '@' && o.reg != 'f') {
+                        // cube ops (sord/input_output.f90:246-251, set_cube :546-591)
+                        // on the true node coordinates or cell centres; the
+                        // reference tests arrays that hold other data there
+                        // (defect D9)
+                        HArr<T> *X = o.reg == 'c' ? xc : xx;
+                        for (int l = x.i1[2]; l <= x.i2[2]; l += x.di[2])
+                            for (int k = x.i1[1]; k <= x.i2[1]; k += x.di[1])
+                                for (int j = x.i1[0]; j <= x.i2[0]; j += x.di[0]) {
+                                    bool in = true;
+                                    for (int a = 0; a < 3; a++) {
+                                        const T c = X[a](j, k, l);
+                                        if (!(c >= o.x1[a] && c <= o.x2[a])) in = false;
+                                    }
+                                    if (!in) continue;
+                                    if (op[0] == '=') f(j, k, l) = val;
+                                    else if (op[0] == '+') f(j, k, l) = f(j, k, l) + val;
+                                    else f(j, k, l) = f(j, k, l) * val;
+                                }
                     } else {
-                        fail(SORD_EUNSUPPORTED, "i/o mode '" + op + "' (random and cube ops)");
+                        fail(SORD_EUNSUPPORTED, "i/o mode '" + op + "' (random ops, cube ops on fault fields)");
                     }
                     bcio(o, f);
                     continue;

@@ -20,9 +20,11 @@
  * Scope (what the reference itself supports without undefined behaviour):
  * rectangular / affine / expanded meshes, all boundary conditions, PML,
  * Kelvin-Voigt viscosity, hourglass control, slip-weakening rupture, and the
- * field-I/O ops '=', '+', '*', '.', '=<', '+<', '*<', '=>', '.>'.  Not
- * restated: random ops ('~', and gridnoise, whose values depend on the
- * Fortran RNG), '=@' cube ops (reference defect D9), '.<' (reads a buffer row
+ * field-I/O ops '=', '+', '*', '.', '=<', '+<', '*<', '=>', '.>', and the
+ * cube ops '=@', '+@', '*@' on volume fields with the intended coordinates
+ * (reference defect D9: parity there is with the solver, not the reference).
+ * Not restated: random ops ('~', and gridnoise, whose values depend on the
+ * Fortran RNG), cube ops on fault fields, '.<' (reads a buffer row
  * with a stale loop index, sord/input_output.f90:350) and debug dumps.  Those abort with exit status 3.  The finite kinematic
  * source (sord/kinematic_source.f90) is restated.
  */
@@ -969,6 +971,33 @@ static void field_io(const char *passes, const char *field, A3 f) {
                     for (int l = i1[2]; l <= i2[2]; l += di[2])
                         for (int k = i1[1]; k <= i2[1]; k += di[1])
                             for (int j = i1[0]; j <= i2[0]; j += di[0]) {
+                                if (io->op[0] == '=') AT(f, j, k, l) = val;
+                                else if (io->op[0] == '+') AT(f, j, k, l) = AT(f, j, k, l) + val;
+                                else AT(f, j, k, l) = AT(f, j, k, l) * val;
+                            }
+                } else if (io->op[1] == '@' && io->reg != 'f') {
+                    /* cube ops: the op on the nodes (cells) of the range whose
+                       coordinates (centres) lie in the box [x1, x2].  The
+                       reference's set_cube (sord/input_output.f90:546-591,
+                       called at :246-251) tests w1 for node fields, which holds
+                       the cell centres at initialisation, and w2 for cell
+                       fields, which holds no coordinates (defect D9); the
+                       intended coordinates are used here, the cell centres by
+                       the reference's own 8-node average (average, :79-101). */
+                    A3 x[3] = {comp(xx, 1), comp(xx, 2), comp(xx, 3)};
+                    for (int l = i1[2]; l <= i2[2]; l += di[2])
+                        for (int k = i1[1]; k <= i2[1]; k += di[1])
+                            for (int j = i1[0]; j <= i2[0]; j += di[0]) {
+                                int in = 1;
+                                for (int a = 0; a < 3; a++) {
+                                    real c = io->reg == 'n' ? AT(x[a], j, k, l)
+                                        : R(0.125) * (AT(x[a], j, k, l) + AT(x[a], j + 1, k + 1, l + 1) +
+                                                      AT(x[a], j, k + 1, l + 1) + AT(x[a], j + 1, k, l) +
+                                                      AT(x[a], j + 1, k, l + 1) + AT(x[a], j, k + 1, l) +
+                                                      AT(x[a], j + 1, k + 1, l) + AT(x[a], j, k, l + 1));
+                                    if (!(c >= io->x1[a] && c <= io->x2[a])) in = 0;
+                                }
+                                if (!in) continue;
                                 if (io->op[0] == '=') AT(f, j, k, l) = val;
                                 else if (io->op[0] == '+') AT(f, j, k, l) = AT(f, j, k, l) + val;
                                 else AT(f, j, k, l) = AT(f, j, k, l) * val;

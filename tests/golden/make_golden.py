@@ -11,6 +11,11 @@ at their own sizes, whose definitions depend on the precision; the two
 16 M-node meshes take 10 to 50 minutes each on 8 cores).
 
     python tests/golden/make_golden.py [--r4 | --r8] [case ...]
+    python tests/golden/make_golden.py --material
+
+--material writes tests/golden/material_json.json: the material.json text the
+reference writes at initialisation (its list-directed layout) for every case
+of CASES, in FP32 and FP64.
 """
 import glob
 import os
@@ -57,8 +62,34 @@ def run_reference(case, realsize, exe):
         shutil.rmtree(d)
 
 
+def material_texts():
+    out = {}
+    for case in cases.CASES:
+        out[case] = {}
+        for rs in (4, 8):
+            exe = os.path.join(ROOT, 'oracle', '_ref', 'r%d' % rs, 'sord.x')
+            prm, files = cases.CASES[case]()
+            prm['shape'] = prm['shape'][:3] + [1]
+            d = tempfile.mkdtemp(prefix='golden-')
+            cwd = os.getcwd()
+            try:
+                os.chdir(d)
+                cases.write_inputs(files, rs)
+                sord.stage(prm, realsize=rs)
+                subprocess.run([exe], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                out[case][str(rs)] = open('material.json').read()
+            finally:
+                os.chdir(cwd)
+                shutil.rmtree(d)
+    import json
+    with open(os.path.join(HERE, 'material_json.json'), 'w') as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+
+
 def main():
     args = sys.argv[1:]
+    if '--material' in args:
+        return material_texts()
     sizes = [4, 8]
     if '--r4' in args or '--r8' in args:
         sizes = [rs for rs in sizes if '--r%d' % rs in args]

@@ -64,3 +64,32 @@ def test_reference_kostrov_run_matches_closed_form():
             v = kostrov.slip_rate(2670.0, 6000.0, 3464.0, vr, 10e6, r, (nt - 1.5) * dt - r / vr)
             err = (ref['p20' + p][-1] - v) / v
             assert abs(err) < 0.03, (realsize, p, err)
+
+
+@pytest.mark.parametrize('realsize', [4, 8])
+def test_oracle_cube_ops_match_closed_form(realsize):
+    """Cube ops ('=@', '*@', '+@', tests/cases.py cube_ops): the cells whose
+    centres lie in each box, from the mesh's closed-form cell centres, get
+    the op in the order listed (the material written at initialisation)."""
+    dt = np.float32 if realsize == 4 else np.float64
+    out = run_case('cube_ops', 'oracle', realsize)
+    nx, ny, nz = cases.cube_ops()[0]['shape'][:3]
+    c = [(np.arange(n - 1) + 0.5) * 100.0 for n in (nx, ny, nz)]
+    X, Y, Z = np.meshgrid(*c, indexing='ij')
+
+    def inbox(b):
+        return ((X >= b[0][0]) & (X <= b[1][0]) & (Y >= b[0][1]) & (Y <= b[1][1]) &
+                (Z >= b[0][2]) & (Z <= b[1][2]))
+    rho = np.full(X.shape, dt(2670.0))
+    rho[inbox(cases.CUBE_BOXES[0])] = dt(2500.0)
+    rho[inbox(cases.CUBE_BOXES[1])] *= dt(0.98)
+    vs = np.full(X.shape, dt(3464.0))
+    vs[inbox(cases.CUBE_BOXES[2])] += dt(-150.0)
+    gam = np.zeros(X.shape, dt)
+    m = inbox(cases.CUBE_BOXES[3])
+    m[:1] = m[12:] = False  # the op's index range: cells 1..11 along x
+    gam[m] = dt(0.02)
+    for name, want in (('rho', rho), ('vs', vs), ('gam', gam)):
+        got = out['out-' + name].reshape(nz - 1, ny - 1, nx - 1).T
+        assert np.array_equal(got, want), name
+    assert (rho != 2670.0).sum() > 500 and m.sum() > 500
