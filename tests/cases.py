@@ -8,6 +8,8 @@ second.  Fault parameters are read from files ('=<'): the reference's
 non-disk ops on fault fields write outside the fault arrays (defect D14), so
 disk input is the form both codes execute the same way.
 """
+import os
+
 import numpy as np
 
 
@@ -452,6 +454,53 @@ def material_ops():
     return prm, files
 
 
+# a kinematic rupture in Standard Rupture Format: two by two subfaults (lon,
+# lat given in grid units, depth in km), three with two slip components, one
+# with a normal component
+SRF_CASE = """2.0
+PLANE 1
+ 0.0 0.0 2 2 1.0 0.5
+ 30.0 60.0 0.5 0.4 0.6
+POINTS 4
+ 8.3 7.6 0.62 30.0 60.0 1.0e9 0.000 0.010
+ 90.0 10.0 3 5.0 2 0.0 0
+ 80.0 120.0 40.0
+ -30.0 -50.0
+ 9.1 7.6 0.62 30.0 60.0 1.0e9 0.015 0.010
+ 45.0 20.0 2 4.0 1 3.0 2
+ 60.0 90.0
+ 70.0
+ 20.0 10.0
+ 8.3 8.4 0.71 30.0 60.0 1.0e9 0.030 0.005
+ 0.0 30.0 4 0.0 0 0.0 0
+ 50.0 100.0 100.0 50.0
+ 9.1 8.4 0.71 30.0 60.0 1.0e9 0.045 0.010
+ 180.0 40.0 1 2.0 1 0.0 0
+ 90.0
+ 60.0
+"""
+
+
+def srf_potency():
+    """The finite potency source written from an SRF rupture by
+    coseis_amd.srf.write_sord (8 point sources: each subfault's slip
+    components with samples), free top, PML elsewhere."""
+    import io
+    import tempfile
+    from coseis_amd import srf
+    prm, _ = _finite('potency')
+    d = tempfile.mkdtemp(prefix='srf-')
+    n = srf.write_sord(os.path.join(d, 'source-'), srf.read(io.StringIO(SRF_CASE)), delta=(1.0, 1.0, 100.0),
+                       dbytes=8)
+    files = {}
+    for name in os.listdir(d):
+        files[name] = np.fromfile(os.path.join(d, name), '<i4' if name == 'source-nt.bin' else '<f8')
+        os.remove(os.path.join(d, name))
+    os.rmdir(d)
+    prm['nsource'] = n
+    return prm, files
+
+
 CASES = {
     'explosion_pml': explosion_pml,
     'layered_free': layered_free,
@@ -472,6 +521,7 @@ CASES = {
     'pulses': pulses,
     'fault_opening': fault_opening,
     'material_ops': material_ops,
+    'srf_potency': srf_potency,
 }
 
 

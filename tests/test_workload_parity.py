@@ -57,7 +57,9 @@ def run_workload(name, realsize):
 
 
 def compare(name, prm, out, ref, realsize):
-    check_outputs(name, out, ref, golden(name, 8), realsize, dt=prm['delta'][3])
+    path = os.path.join(os.path.dirname(__file__), 'golden', '%s_r8.npz' % name)
+    ref64 = golden(name, 8) if os.path.exists(path) else None
+    check_outputs(name, out, ref, ref64, realsize, dt=prm['delta'][3])
 
 
 @pytest.mark.parametrize('realsize', [4, 8])
