@@ -17,13 +17,17 @@ over the K steps (nodes nx*ny*nz per step, halo excluded).
 roofline: the fused interior kernel (k_fused) is the dominant kernel; its
 algorithmic bytes per node-update are 68 B in FP32 (read u, v: 24 B, write u,
 v: 24 B, cell lambda, mu, yy: 12 B, node 1/m, gamma: 8 B); one launch advances
-the interior box, so achieved = 68 B x nodes / mean launch duration (HIP
-events around each launch on the solver stream, on which every kernel of the
-step runs in order).  traffic: HBM bytes per launch from rocprofv3
+the interior box, so achieved = 68 B x nodes / mean launch duration (a pair of
+HIP events per launch that the launch itself stamps with the kernel's start
+and end, hipExtLaunchKernelGGL, on the solver stream, on which every kernel of
+the step runs in order).  traffic: HBM bytes per launch from rocprofv3
 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy with the same
 4-byte access width), read from the committed summary in profiles/.
 step_roofline: the whole step, 68 B per node plus 48 B per node and damped
-axis (PML memory g and p), over the measured step time.
+axis (PML memory g and p), over the measured step time.  roofline.boundary:
+each boundary launch of a step on the same bytes (its nodes and their damped
+axes) over its mean duration, from --boundary-steps further steps with every
+boundary launch stamping a pair of HIP events the same way.
 
 batched_vs_plain: the batched ensemble (64 seeded scenarios, 48..160 nodes
 per edge, PML, FP32; batched_leg) against the same scenarios run one after
@@ -84,6 +88,8 @@ def parse():
     p.add_argument('--realsize', type=int, default=4)
     p.add_argument('--workload', default='layered', choices=sorted(WORKLOADS),
                    help='plain-forward workload (layered: the metric; explosion, rupture: the other configurations)')
+    p.add_argument('--boundary-steps', type=int, default=100,
+                   help='steps after the timed ones with every boundary launch timed (0: none)')
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--cpu-steps', type=int, default=6)
     p.add_argument('--stage-only', action='store_true', help='stage kernels only (no fused kernel)')
@@ -284,6 +290,23 @@ def adjoint_leg(n, nt):
     }
 
 
+def boundary_rooflines(launches, realsize, steps):
+    """Per boundary launch of a step (k_fusedp, one damped-axis mask each):
+    algorithmic bytes = 68 B per node (FP32) + 48 B per node and damped axis,
+    over the launch's mean duration (HIP events that the launch stamps with
+    the kernel's start and end, on the solver stream, over `steps` steps run
+    after the timed ones)."""
+    out = []
+    for mask, ms, nodes, damped in launches:
+        alg = ALG_BYTES[realsize] * nodes + PML_BYTES[realsize] * damped
+        gbs = alg / (ms / 1e3) / 1e9 if ms > 0 else None
+        out.append({'kernel': 'k_fusedp mask %d' % mask, 'nodes': int(nodes), 'damped_memberships': int(damped),
+                    'alg_bytes': int(alg), 'launch_ms': round(ms, 5),
+                    'achieved': round(gbs, 1) if gbs else None,
+                    'frac': round(gbs / HBM_PEAK_GBS, 4) if gbs else None, 'steps': steps})
+    return out
+
+
 def progress(msg):
     print('bench: ' + msg, file=sys.stderr, flush=True)
 
@@ -299,7 +322,7 @@ def main():
         sys.exit(2)
     import torch
     from coseis_amd import configs, solver, sord
-    nt = a.warmup + a.steps
+    nt = a.warmup + a.steps + a.boundary_steps
     prm, files = getattr(configs, WORKLOADS[a.workload][0])(nt=nt)
     nx, ny, nz = prm['shape'][:3]
     d = tempfile.mkdtemp(prefix='sord-bench-')
@@ -317,6 +340,7 @@ def main():
         ms_total, ms_kernel, knodes, kname = s.bench(a.steps)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        bnd = s.bench_boundary(a.boundary_steps) if a.boundary_steps and not a.stage_only else []
         s.close()
     finally:
         os.chdir(cwd)
@@ -354,6 +378,8 @@ def main():
         'batched_vs_plain': None,
         'grad_over_forward': None,
     }
+    if bnd:
+        line['roofline']['boundary'] = boundary_rooflines(bnd, a.realsize, a.boundary_steps)
     progress('plain forward: %.3f Gnode/s' % value)
     if not a.no_batch:
         line['batched_vs_plain'], line['batched'] = batched_leg(a.batch_scenarios, a.batch_steps)

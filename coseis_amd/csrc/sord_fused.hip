@@ -30,6 +30,8 @@
 // the column j-1 part through a lane shuffle.  u, v, gamma, 1/m and
 // lambda, mu, yy are read once and u, v written once per node: 68 bytes per
 // node-update in FP32 (plus the PML memory in the slabs).
+#include <hip/hip_ext.h>
+
 #include "sord_dev.h"
 #include "sord_launch.h"
 
@@ -1019,40 +1021,37 @@ FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work) {
 
 template <class T>
 void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3],
-                  int waves, bool fs, hipStream_t s) {
+                  int waves, bool fs, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (nwork <= 0) return;
     const FusedC<T> C = fused_consts<T>(P, sp, work);
-    const dim3 g(nwork), b8(64, 8), bb(64, BOUNDARY_WAVES);
-    if (fs) {
-        if (mask >= 0) hipLaunchKernelGGL((k_fusedp_fs<T, BOUNDARY_WAVES>), g, bb, 0, s, P, C, ops);
-        else if (waves == 4) hipLaunchKernelGGL((k_fused_fs<T, 4>), g, dim3(64, 4), 0, s, P, C, ops);
-        else if (waves == 8) hipLaunchKernelGGL((k_fused_fs<T, 8>), g, b8, 0, s, P, C, ops);
-        else hipLaunchKernelGGL((k_fused_fs<T, 16>), g, dim3(64, 16), 0, s, P, C, ops);
-        return;
-    }
+    using K = void (*)(Params<T>, FusedC<T>, PointOps<T>);
     const bool st = P.stat_now || P.stat_next;
-    switch (mask) {
-    case -1:
-        if (st) switch (waves) {
-            case 4: hipLaunchKernelGGL((k_fused_stat<T, 4>), g, dim3(64, 4), 0, s, P, C, ops); break;
-            case 8: hipLaunchKernelGGL((k_fused_stat<T, 8>), g, b8, 0, s, P, C, ops); break;
-            default: hipLaunchKernelGGL((k_fused_stat<T, 16>), g, dim3(64, 16), 0, s, P, C, ops); break;
-            }
-        else switch (waves) {
-            case 4: hipLaunchKernelGGL((k_fused<T, 4>), g, dim3(64, 4), 0, s, P, C, ops); break;
-            case 8: hipLaunchKernelGGL((k_fused<T, 8>), g, b8, 0, s, P, C, ops); break;
-            default: hipLaunchKernelGGL((k_fused<T, 16>), g, dim3(64, 16), 0, s, P, C, ops); break;
-            }
-        break;
-    case 0: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 0>), g, bb, 0, s, P, C, ops); break;
-    case 1: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 1>), g, bb, 0, s, P, C, ops); break;
-    case 2: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 2>), g, bb, 0, s, P, C, ops); break;
-    case 3: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 3>), g, bb, 0, s, P, C, ops); break;
-    case 4: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 4>), g, bb, 0, s, P, C, ops); break;
-    case 5: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 5>), g, bb, 0, s, P, C, ops); break;
-    case 6: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 6>), g, bb, 0, s, P, C, ops); break;
-    default: hipLaunchKernelGGL((k_fusedp<T, BOUNDARY_WAVES, 7>), g, bb, 0, s, P, C, ops); break;
+    K k;
+    int w = mask >= 0 ? BOUNDARY_WAVES : waves == 4 || waves == 8 ? waves : 16;
+    if (fs) {
+        k = mask >= 0 ? k_fusedp_fs<T, BOUNDARY_WAVES>
+            : w == 4  ? k_fused_fs<T, 4>
+            : w == 8  ? k_fused_fs<T, 8>
+                      : k_fused_fs<T, 16>;
+    } else if (mask < 0) {
+        k = st ? (w == 4 ? k_fused_stat<T, 4> : w == 8 ? k_fused_stat<T, 8> : k_fused_stat<T, 16>)
+               : (w == 4 ? k_fused<T, 4> : w == 8 ? k_fused<T, 8> : k_fused<T, 16>);
+    } else {
+        switch (mask) {
+        case 0: k = k_fusedp<T, BOUNDARY_WAVES, 0>; break;
+        case 1: k = k_fusedp<T, BOUNDARY_WAVES, 1>; break;
+        case 2: k = k_fusedp<T, BOUNDARY_WAVES, 2>; break;
+        case 3: k = k_fusedp<T, BOUNDARY_WAVES, 3>; break;
+        case 4: k = k_fusedp<T, BOUNDARY_WAVES, 4>; break;
+        case 5: k = k_fusedp<T, BOUNDARY_WAVES, 5>; break;
+        case 6: k = k_fusedp<T, BOUNDARY_WAVES, 6>; break;
+        default: k = k_fusedp<T, BOUNDARY_WAVES, 7>; break;
+        }
     }
+    const dim3 g(nwork), b(64, w);
+    // with events: the runtime stamps the kernel's own start and end into them
+    if (e0) hipExtLaunchKernelGGL(k, g, b, 0, s, e0, e1, 0, P, C, ops);
+    else hipLaunchKernelGGL(k, g, b, 0, s, P, C, ops);
 }
 
 template <class T>
@@ -1096,8 +1095,8 @@ template FusedC<double> fused_consts<double>(const Params<double> &, const doubl
 template void launch_fused_batch<float>(const Scen<float> *, const Work *, int, int, bool, hipStream_t);
 template void launch_fused_batch<double>(const Scen<double> *, const Work *, int, int, bool, hipStream_t);
 template void launch_fused<float>(const Params<float> &, const PointOps<float> &, const Work *, int, int,
-                                  const float *, int, bool, hipStream_t);
+                                  const float *, int, bool, hipStream_t, hipEvent_t, hipEvent_t);
 template void launch_fused<double>(const Params<double> &, const PointOps<double> &, const Work *, int, int,
-                                   const double *, int, bool, hipStream_t);
+                                   const double *, int, bool, hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace sord

@@ -1026,6 +1026,30 @@ extern "C" int sord_bench(void *h, int nsteps, double *ms_total, double *ms_kern
     });
 }
 
+extern "C" int sord_bench_boundary(void *h, int nsteps, int *nlaunch, int *masks, double *ms, double *nodes,
+                                   double *damped, int cap) {
+    return dispatch(h, [&](auto &S) {
+        if (!S.initialized) fail(SORD_ESTATE, "sord_bench_boundary before sord_init");
+        if (S.use_side) fail(SORD_ESTATE, "sord_bench_boundary: boundary launches on side streams");
+        double tot = 0, kern = 0;
+        S.bench_bnd_on = true;
+        try {
+            S.bench(nsteps, &tot, &kern, nullptr, 0);
+        } catch (...) {
+            S.bench_bnd_on = false;
+            throw;
+        }
+        S.bench_bnd_on = false;
+        *nlaunch = (int)S.blaunch.size();
+        for (int q = 0; q < *nlaunch && q < cap; q++) {
+            masks[q] = S.blaunch[q].mask;
+            ms[q] = q < (int)S.bench_bnd_ms.size() ? S.bench_bnd_ms[q] : 0.0;
+            nodes[q] = S.blaunch[q].nodes;
+            damped[q] = S.blaunch[q].damped;
+        }
+    });
+}
+
 extern "C" int sord_copy_probe(const void *src, void *dst, int64_t nbytes, int device) {
     return guard([&]() {
         if (nbytes < 0 || nbytes % 4) fail(SORD_EINPUT, "copy probe: a whole number of 4-byte words");

@@ -30,7 +30,8 @@ template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, h
 namespace sord {
 // mask -1: interior kernel with `waves` waves per block; 0..7: boundary kernel
 // specialised on the damped axes (bit a: axis a)
-template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, bool fs, hipStream_t s);
+// e0, e1: when given, the kernel's own start and end times are recorded in them
+template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops, const Work *work, int nwork, int mask, const T sp[3], int waves, bool fs, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // constants of the fused kernels for one scenario; the batch launch takes a
 // device table of scenarios, each work item naming its own (Work.pad1)
 template <class T> int boundary_blocks_per_cu();

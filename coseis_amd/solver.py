@@ -18,7 +18,7 @@ ERRORS = {
 EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_step_static', 'sord_run', 'sord_status', 'sord_field', 'sord_set_tensor',
     'sord_tensor_numel',
-    'sord_bench', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_plan', 'sord_copy_probe',
+    'sord_bench', 'sord_bench_boundary', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_plan', 'sord_copy_probe',
     'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_streams', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
     'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
     'sord_adjoint_setup', 'sord_adjoint_forward', 'sord_adjoint_backward', 'sord_adjoint_timing',
@@ -59,6 +59,9 @@ def load():
     lib.sord_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_char_p, ctypes.c_int]
     lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.sord_bench_boundary.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        dp, dp, dp, ctypes.c_int]
     lib.sord_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.sord_set_streams.argtypes = [vp, ctypes.c_int]
     lib.sord_plan.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -174,6 +177,17 @@ class Solver(object):
         _check(_lib.sord_bench(self._h, nsteps, ctypes.byref(tot), ctypes.byref(ker), ctypes.byref(nodes),
                                name, 64))
         return tot.value, ker.value, nodes.value, name.value.decode()
+
+    def bench_boundary(self, nsteps):
+        """Advance nsteps steps timing each boundary launch (one stream):
+        [(mask, mean ms, nodes, damped-axis memberships)] per launch of a step
+        (sord_bench_boundary)."""
+        n = ctypes.c_int()
+        cap = 64
+        m = (ctypes.c_int * cap)()
+        ms, nodes, damped = (ctypes.c_double * cap)(), (ctypes.c_double * cap)(), (ctypes.c_double * cap)()
+        _check(_lib.sord_bench_boundary(self._h, nsteps, ctypes.byref(n), m, ms, nodes, damped, cap))
+        return [(m[i], ms[i], nodes[i], damped[i]) for i in range(min(n.value, cap))]
 
     # ---- adjoint gradient (sord_adjoint_*) ----
     def adjoint_setup(self, receivers, every):

@@ -101,11 +101,22 @@ int sord_tensor_numel(void *handle, int64_t *numel);
 
 /* Advance `nsteps` steps (with their field i/o) timed with HIP events on the
  * library's stream.  Returns the elapsed time of all steps (ms), the mean
- * duration of one launch of the dominant solver kernel (ms), the number of
+ * duration of one launch of the dominant solver kernel (ms; a pair of events
+ * that each launch stamps with the kernel's start and end), the number of
  * nodes that launch advances, and the kernel's name.  No counterpart in the
  * reference: it is the measurement hook of the benchmark. */
 int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, int64_t *kernel_nodes,
                char *kernel_name, int name_len);
+
+/* Advance nsteps steps as sord_bench does, with every boundary launch
+ * stamping a pair of HIP events on the solver stream (one stream only: SORD_ESTATE
+ * with side streams).  Returns the number of boundary launches per step and,
+ * in [0, cap), each one's damped-axis specialisation, mean duration (ms), the
+ * nodes it advances and the sum over them of the axes along which the node is
+ * damped (the PML memory it reads and writes).  No counterpart in the
+ * reference: the measurement hook of the boundary kernels' roofline. */
+int sord_bench_boundary(void *handle, int nsteps, int *nlaunch, int *masks, double *ms, double *nodes,
+                        double *damped, int cap);
 
 /* Select the execution path: 1 fused interior kernel where it applies
  * (default), 0 stage kernels everywhere. */
