@@ -112,12 +112,16 @@ def group_rel_l2(out, ref, grp):
 def rupture_times_agree(name, out, ref, dt, realsize):
     """Rupture time (trup) or arrest time (tarr) where the reference's fault
     node ruptured (trup < 1e8) or arrested (0 < tarr < 1e8): the same set of
-    nodes, and times within one time step in FP32 or 1e-9 s in FP64.
-    Returns the largest difference."""
+    nodes, and times within one time step in FP32 or 1e-9 s in FP64.  Every
+    rupture fixture ruptures somewhere; where the reference's run was too
+    short for any node to arrest, none may arrest.  Returns the largest
+    difference."""
     hit = (ref < 1e8) & (ref > 0 if name == 'tarr' else True)
     mine = (out < 1e8) & (out > 0 if name == 'tarr' else True)
-    assert hit.any(), name
     assert np.array_equal(mine, hit), (name, int(mine.sum()), int(hit.sum()), int((mine != hit).sum()))
+    if name == 'tarr' and not hit.any():
+        return 0.0
+    assert hit.any(), name
     bar = dt if realsize == 4 else 1e-9
     err = float(np.abs(out[hit].astype('f8') - ref[hit]).max())
     assert err <= bar, (name, err, bar)
@@ -138,6 +142,8 @@ def check_outputs(case, out, ref, ref64, realsize, dt=None):
     if dt is None:
         dt = cases.CASES[case]()[0]['delta'][3]
     errs, bad, noisy = {}, {}, []
+    if ref64 is not None and any(k in ref64 and ref64[k].shape != ref[k].shape for k in ref if k.startswith('stats-')):
+        ref64 = None  # the FP64 fixture covers another step window: no conditioning estimate
     for grp in vector_groups(set(ref)):
         for k in grp:
             assert out[k].shape == ref[k].shape, k
@@ -149,8 +155,7 @@ def check_outputs(case, out, ref, ref64, realsize, dt=None):
             bar = ORDER_FREE.get(grp[0], TOL[8])
             err = group_rel_l2(out, ref, grp)
         else:
-            # the reference's own FP32-vs-FP64 distance (none where the FP64
-            # fixture covers another step window)
+            # the reference's own FP32-vs-FP64 distance
             same = ref64 is not None and all(g in ref64 and ref64[g].shape == ref[g].shape for g in grp)
             cond = group_rel_l2(ref, ref64, grp) if same else 0.0
             if cond <= TOL[4]:
