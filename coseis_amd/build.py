@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libsord_amd.so')
-SOURCES = ['sord_kernels.hip', 'sord_fused.hip', 'sord_adjoint.hip', 'sord_host.cpp']
+SOURCES = ['sord_kernels.hip', 'sord_fused.hip', 'sord_adjoint.hip', 'sord_init.hip', 'sord_host.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = [
     '-std=c++17', '-O3', '--offload-arch=gfx950', '-fPIC',

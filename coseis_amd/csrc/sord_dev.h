@@ -27,6 +27,19 @@ struct Grid {
     }
 };
 
+// limits and constants of the device-side material model
+template <class T>
+struct MatInit {
+    T rho_min, rho_max, vp_min, vp_max, vs_min, vs_max, gam_min, gam_max, vdamp;
+    int3 h1, h2;  // the cells: the halo outside is zeroed
+    static __host__ __device__ constexpr T big() { return sizeof(T) == 4 ? T(3.40282347e38f) : T(1.7976931348623157e308); }
+};
+// seven cell arrays reduced together (rho, vp, vs, gam, lam, mu, nu)
+template <class T>
+struct MatArrays {
+    const T *p[7];
+};
+
 // A small list of per-cell or per-node operations (value already scaled by the
 // time function) applied inside the stress kernel: potency, moment, stress.
 template <class T>

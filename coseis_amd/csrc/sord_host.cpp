@@ -259,6 +259,10 @@ struct Solver {
     T *red_val = nullptr;
     long *red_loc = nullptr;
     bool use_fused = true;
+    // material model, resampling and rectilinear cell volumes computed on the
+    // device from the input arrays (sord_set_device_init); else on the host
+    bool dev_init = true;
+    bool host_mat_ready = false;  // device path: the host copies the adjoint reads are filled
     // boundary launches on side streams beside the interior kernel (opt-in,
     // sord_set_streams) or, by default, in order on the one solver stream:
     // with side streams, outputs were seen to differ between processes
@@ -556,9 +560,12 @@ struct Solver {
     // (sord/input_output.f90:295-437).  Fills lo/hi and vals (Fortran order).
     bool io_read(IoRec<T> &o, Index &x, const int fn[3], int lo[3], int hi[3], std::vector<T> &vals) {
         const int it1 = o.ii[3][0], it2 = o.ii[3][1], dit = o.ii[3][2];
-        if (o.op[0] == '.') fail(SORD_EUNSUPPORTED, "'.<' (the reference reads a stale buffer row, "
-                                                    "sord/input_output.f90:350)");
-        for (int i = 0; i < 3; i++) {
+        // '.<': a point time series, one value per step, spread over the
+        // point's cell or node neighbours (sord/input_output.f90:343-353); the
+        // reference reads buff(i, ib) with i left over from an earlier loop
+        // (:350), the record holds one value: buff(1, ib) here
+        const bool point = o.op[0] == '.';
+        for (int i = 0; i < 3 && !point; i++) {
             // the reference interpolates a broadcast (one-value) axis read at a
             // stride from unread scratch values, and the last stretch of a range
             // whose stop is off the stride grid likewise
@@ -590,6 +597,15 @@ struct Solver {
         }
         o.ib++;
         const T *b = o.hbuf.data() + rec * (size_t)(o.ib - 1);
+        if (point) {  // the box of the '.' op (io_box_nondisk) and the step's value
+            for (int i = 0; i < 3; i++) lo[i] = x.i1[i], hi[i] = x.i2[i];
+            vals.assign(1, b[0]);
+            if (it == it2) {
+                o.hbuf.clear();
+                o.op[0] = '#';
+            }
+            return true;
+        }
         std::vector<T> dense;
         if (x.di[0] > 1 || x.di[1] > 1 || x.di[2] > 1) {
             // the read nodes at their places in the box x.i1..x.i2, then linear
@@ -735,6 +751,17 @@ struct Solver {
                     std::vector<T> vals;
                     const char opc = o.op[0];  // io_read retires the record on its last step
                     if (!io_read(o, x, f.n, lo, hi, vals)) continue;
+                    if (opc == '.') {  // sub-cell interpolation of the point's value
+                        for (int l = lo[2]; l <= hi[2]; l++)
+                            for (int k = lo[1]; k <= hi[1]; k++)
+                                for (int j = lo[0]; j <= hi[0]; j++) {
+                                    T r = (T(1) - std::fabs(x.xi[0] - (T)j)) * (T(1) - std::fabs(x.xi[1] - (T)k)) *
+                                          (T(1) - std::fabs(x.xi[2] - (T)l));
+                                    f(j, k, l) = f(j, k, l) + r * vals[0];
+                                }
+                        bcio(o, f);
+                        continue;
+                    }
                     size_t c = 0;
                     for (int l = lo[2]; l <= hi[2]; l++)
                         for (int k = lo[1]; k <= hi[1]; k++)
@@ -950,9 +977,9 @@ extern "C" int sord_field(void *h, const char *name, void **ptr, int64_t shape[3
         }
         for (int i = 0; i < 6; i++)
             if (n == wn[i]) p = S.P.sig[i];
-        static const char *mn[5] = {"lam", "mu", "yy", "mr", "gam"};
-        void *mp[5] = {S.P.lam, S.P.mu, S.P.yy, S.P.mr, S.P.gam};
-        for (int i = 0; i < 5; i++)
+        static const char *mn[6] = {"lam", "mu", "yy", "mr", "gam", "vc"};
+        void *mp[6] = {S.P.lam, S.P.mu, S.P.yy, S.P.mr, S.P.gam, S.P.vc};
+        for (int i = 0; i < 6; i++)
             if (n == mn[i]) p = mp[i];
         if (!p) fail(SORD_EINPUT, "no device field named '" + n + "'");
         HIPCHECK(hipStreamSynchronize(S.stream));
@@ -972,6 +999,13 @@ extern "C" int sord_tensor_numel(void *h, int64_t *numel) {
 
 extern "C" int sord_set_tensor(void *h, const char *name, void *dev_ptr, int64_t numel) {
     return dispatch(h, [&](auto &S) { S.adopt(name ? name : "", dev_ptr, numel); });
+}
+
+extern "C" int sord_set_device_init(void *h, int on) {
+    return dispatch(h, [on](auto &S) {
+        if (S.initialized) fail(SORD_ESTATE, "sord_set_device_init after sord_init");
+        S.dev_init = on != 0;
+    });
 }
 
 extern "C" int sord_set_fused(void *h, int on) {

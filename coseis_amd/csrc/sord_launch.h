@@ -25,6 +25,15 @@ template <class T> void launch_rupture(const Params<T> &P, const Fault<T> &F, in
 template <class T> void launch_fault_time(const Params<T> &P, const Fault<T> &F, T tm, T svtol, hipStream_t s);
 template <class T> void launch_fault_svold(const Params<T> &P, const Fault<T> &F, hipStream_t s);
 template <class T> void launch_fault_sa(const Params<T> &P, const Fault<T> &F, hipStream_t s);
+// device-side initialisation (sord_init.hip)
+template <class T> void launch_init_vc(T *vc, const Grid<T> &g, const T *hx, const T *hy, const T *hz, const int c1[3], const int c2[3], bool flip, int fa, int irup, hipStream_t s);
+template <class T> void launch_init_cells(const Grid<T> &g, const MatInit<T> &m, const T *rho, const T *vp, const T *vs, T *gam, T *rc, T *s1, T *s2, T *lam, T *mu, T *nu, hipStream_t s);
+template <class T> void launch_init_minmax(const Grid<T> &g, const MatArrays<T> &a, const int lo[3], const int hi[3], T *part, int nblk, hipStream_t s);
+template <class T> void launch_init_yy(const Grid<T> &g, T *yy, const T *lam, const T *mu, T sc, hipStream_t s);
+template <class T> void launch_init_scale(const Grid<T> &g, T *out, const T *a, const T *b, T c, hipStream_t s);
+template <class T> void launch_init_nodeavg(const Grid<T> &g, T *out, const T *in, const int n1[3], const int n2[3], bool inv, hipStream_t s);
+template <class T> void launch_init_recip(const Grid<T> &g, T *a, hipStream_t s);
+template <class T> void launch_init_halo(const Grid<T> &g, T *f, T r, const int i1[3], const int i2[3], int pa, int idx, hipStream_t s);
 }  // namespace sord
 
 namespace sord {

@@ -18,7 +18,7 @@ ERRORS = {
 EXPORTS = [
     'sord_create', 'sord_init', 'sord_step', 'sord_step_static', 'sord_run', 'sord_status', 'sord_field', 'sord_set_tensor',
     'sord_tensor_numel',
-    'sord_bench', 'sord_bench_boundary', 'sord_set_fused', 'sord_set_streams', 'sord_set_tiles', 'sord_plan', 'sord_copy_probe',
+    'sord_bench', 'sord_bench_boundary', 'sord_set_fused', 'sord_set_device_init', 'sord_set_streams', 'sord_set_tiles', 'sord_plan', 'sord_copy_probe',
     'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_streams', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
     'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
     'sord_adjoint_setup', 'sord_adjoint_forward', 'sord_adjoint_backward', 'sord_adjoint_timing',
@@ -59,6 +59,7 @@ def load():
     lib.sord_bench.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_char_p, ctypes.c_int]
     lib.sord_set_fused.argtypes = [vp, ctypes.c_int]
+    lib.sord_set_device_init.argtypes = [vp, ctypes.c_int]
     dp = ctypes.POINTER(ctypes.c_double)
     lib.sord_bench_boundary.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                         dp, dp, dp, ctypes.c_int]
@@ -112,12 +113,14 @@ class Solver(object):
     output files, as the reference ``sord.x`` writes them into its cwd.
     """
 
-    def __init__(self, text, realsize=4, device=0, fused=True, tiles=None, side_streams=False):
+    def __init__(self, text, realsize=4, device=0, fused=True, tiles=None, side_streams=False, device_init=True):
         lib = load()
         self._h = ctypes.c_void_p()
         self._adopted = {}  # caller tensors the handle uses (set_tensor)
         _check(lib.sord_create(text.encode(), realsize, device, ctypes.byref(self._h)))
         self.realsize = realsize
+        if not device_init:
+            _check(lib.sord_set_device_init(self._h, 0))
         if not fused:
             _check(lib.sord_set_fused(self._h, 0))
         if side_streams:

@@ -121,6 +121,14 @@ int sord_bench_boundary(void *handle, int nsteps, int *nlaunch, int *masks, doub
 /* Select the execution path: 1 fused interior kernel where it applies
  * (default), 0 stage kernels everywhere. */
 int sord_set_fused(void *handle, int on);
+
+/* Where sord_init computes the material model (sord/material_model.f90:6-171),
+ * the resampling to nodes (sord/material_resample.f90:6-55) and the volumes
+ * of rectilinear cells (sord/grid_generation.f90:183-198): 1 (the default) on
+ * the device from the input arrays, 0 on the host.  Both give the same arrays
+ * bit for bit.  Before sord_init only (SORD_ESTATE after).  No counterpart in
+ * the reference, which initialises on its CPU. */
+int sord_set_device_init(void *handle, int on);
 /* side = 1: the boundary kernels run on three side streams beside the
  * interior kernel (forked and joined with events); side = 0 (the default):
  * in order on the handle's stream after the interior kernel.  The kernels

@@ -20,13 +20,15 @@
  * Scope (what the reference itself supports without undefined behaviour):
  * rectangular / affine / expanded meshes, all boundary conditions, PML,
  * Kelvin-Voigt viscosity, hourglass control, slip-weakening rupture, and the
- * field-I/O ops '=', '+', '*', '.', '=<', '+<', '*<', '=>', '.>', and the
+ * field-I/O ops '=', '+', '*', '.', '=<', '+<', '*<', '=>', '.>', the
  * cube ops '=@', '+@', '*@' on volume fields with the intended coordinates
- * (reference defect D9: parity there is with the solver, not the reference).
- * Not restated: random ops ('~', and gridnoise, whose values depend on the
- * Fortran RNG), cube ops on fault fields, '.<' (reads a buffer row
- * with a stale loop index, sord/input_output.f90:350) and debug dumps.  Those abort with exit status 3.  The finite kinematic
- * source (sord/kinematic_source.f90) is restated.
+ * (reference defect D9: parity there is with the solver, not the reference),
+ * and '.<' with the intended buffer row (the reference reads it with a stale
+ * loop index, sord/input_output.f90:350: parity there is with the solver and
+ * the '.' op's closed form).  Not restated: random ops ('~', and gridnoise,
+ * whose values depend on the Fortran RNG), cube ops on fault fields and
+ * debug dumps.  Those abort with exit status 3.  The finite kinematic source
+ * (sord/kinematic_source.f90) is restated.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1059,8 +1061,6 @@ static void field_io(const char *passes, const char *field, A3 f) {
                             n[i] = 1;
                             o[i] = 0;
                         }
-                } else {
-                    die(3, "oracle: '.<' is not restated (reference defect)");
                 }
                 size_t rec = (size_t)n[0] * n[1] * n[2];
                 if (io->ib < 0) {
@@ -1089,6 +1089,20 @@ static void field_io(const char *passes, const char *field, A3 f) {
                 }
                 io->ib++;
                 real *b = io->buff + rec * (size_t)(io->ib - 1);
+                if (!strcmp(io->op, ".<")) {
+                    /* sub-cell interpolation of the step's value of a point
+                       time series (sord/input_output.f90:343-353).  Reference
+                       defect: it reads buff(i, ib) with i left over from an
+                       earlier loop (:350); the record holds one value, read
+                       here as buff(1, ib). */
+                    for (int l = i1[2]; l <= i2[2]; l++)
+                        for (int k = i1[1]; k <= i2[1]; k++)
+                            for (int j = i1[0]; j <= i2[0]; j++) {
+                                real r = (R(1.0) - FABS(xi[0] - (real)j)) * (R(1.0) - FABS(xi[1] - (real)k)) *
+                                         (R(1.0) - FABS(xi[2] - (real)l));
+                                AT(f, j, k, l) = AT(f, j, k, l) + r * b[0];
+                            }
+                } else {
                 size_t c = 0;
                 for (int l = i1[2]; l <= i2[2]; l += di[2])
                     for (int k = i1[1]; k <= i2[1]; k += di[1])
@@ -1125,6 +1139,7 @@ static void field_io(const char *passes, const char *field, A3 f) {
                             else if (!strcmp(io->op, "*<")) AT(f, j, k, l) = AT(f, j, k, l) * AT(s1, j, k, l);
                             else die(1, "bad i/o op");
                         }
+                }
                 if (io->reg == 'n') scalar_bc(f, bc1, bc2, i1bc, i2bc);
                 if (io->reg == 'c') {
                     int t[3] = {i2bc[0] - 1, i2bc[1] - 1, i2bc[2] - 1};

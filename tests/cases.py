@@ -566,9 +566,57 @@ CUBE_BOXES = (([320.0, 180.0, 140.0], [880.0, 720.0, 660.0]),
               ([470.0, 260.0, 240.0], [1160.0, 840.0, 760.0]),
               ([20.0, 20.0, 320.0], [1500.0, 1300.0, 1100.0]))
 
+# the points and pulses of point_series: (fields, point, value, pulse, tau,
+# stage delay in steps: the moment enters at the stress stage, the force at
+# the acceleration stage of the same step)
+SERIES = ((('mxx', 'myy', 'mzz'), [7.3, 6.6, 5.2], 1e16, 'brune', 0.04),
+          (('fz',), [9.4, 8.2, 6.7], 1e11, 'gaussian', 0.03))
+
+
+def _pulse(name, t, tau):
+    """The time functions of SERIES (sord/utilities.f90:223-282) in FP64."""
+    t = np.asarray(t, 'd')
+    if name == 'brune':
+        a = 1.0 / tau
+        return np.where(t > 0, np.exp(-a * t) * a * a * t, 0.0)
+    a = 0.5 / (tau * tau)
+    return np.exp(-a * t * t) * np.sqrt(a / 3.14159265)
+
+
+def point_series(series=True):
+    """A moment and a point force read as time series from files ('.<': one
+    value per step, spread over the point's 8 cells or nodes with trilinear
+    weights), or with series=False the same sources as '.' ops with their
+    pulses.  The files hold the pulse at each step's time, so both runs
+    agree.  The reference reads the series from a stale buffer row (defect
+    of sord/input_output.f90:350): parity is with the oracle, which reads the
+    intended row, and between the two forms."""
+    n, nt, dt = 16, 40, 0.0075
+    prm = {
+        'shape': [n, n, n, nt], 'delta': [100.0, 100.0, 100.0, dt],
+        'bc1': ['pml'] * 3, 'bc2': ['pml'] * 3, 'npml': 3,
+        'rho': [2670.0], 'vp': [6000.0], 'vs': [3464.0], 'gam': [0.0],
+        'itstats': 5, 'itio': 16,
+    }
+    files = {}
+    tm = dt * np.arange(nt)  # the time of step it + 1 at its stress and acceleration stages
+    for fields, x, val, pulse, tau in SERIES:
+        for f in fields:
+            if series:
+                name = 'series-%s.bin' % f
+                files[name] = val * _pulse(pulse, tm, tau)
+                prm[f] = [(list(x) + [[]], '.<', name)]
+            else:
+                prm[f] = [(list(x) + [[]], '.', val, pulse, tau)]
+    prm['vx'] = [([[], [], [], -1], '=>', 'vx.bin'), ([11.0, 9.0, 6.0, []], '.>', 'p1-vx.bin')]
+    prm['vz'] = [([[], [], [], -1], '=>', 'vz.bin'), ([5, 10, 8, []], '=>', 'p2-vz.bin')]
+    return prm, files
+
+
 # cases whose parity is with the oracle, not the reference (intended semantics
 # where the reference's own are undefined)
-ORACLE_CASES = {'cube_ops': cube_ops}
+ORACLE_CASES = {'cube_ops': cube_ops, 'point_series': point_series,
+                'point_pulses': lambda: point_series(series=False)}
 
 # ---------------------------------------------------------------------------
 # The benchmark workloads at their own mesh sizes (coseis_amd.configs), with

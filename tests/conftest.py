@@ -46,7 +46,8 @@ def collect(d, files, realsize):
     return out
 
 
-def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, side_streams=False, probe=None):
+def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, side_streams=False, probe=None,
+             device_init=True):
     """Stage a case in a scratch directory and run it; returns {name: array}.
     probe(solver), if given, is called on the HIP solver before the run."""
     prm, files = (cases.CASES.get(case) or cases.ORACLE_CASES[case])()
@@ -63,7 +64,8 @@ def run_case(case, runner, realsize=4, prm_update=None, fused=True, tiles=None, 
             subprocess.run([oracle_exe(realsize)], check=True, stdout=subprocess.DEVNULL, env=env)
         elif runner == 'hip':
             from coseis_amd import solver
-            with solver.Solver(text, realsize=realsize, fused=fused, tiles=tiles, side_streams=side_streams) as s:
+            with solver.Solver(text, realsize=realsize, fused=fused, tiles=tiles, side_streams=side_streams,
+                               device_init=device_init) as s:
                 if probe:
                     probe(s)
                 s.run()

@@ -199,3 +199,19 @@ def test_cube_ops_match_oracle(realsize):
         else:
             bar = {4: 1e-4, 8: 1e-9}[realsize]
             assert rel_l2(out[k], ref[k]) <= bar, (k, rel_l2(out[k], ref[k]))
+
+
+@pytest.mark.parametrize('realsize', [4, 8])
+def test_point_series_match_oracle(realsize):
+    """'.<' (a moment and a point force read as per-step series from files,
+    tests/cases.py point_series) against the oracle, which reads the
+    intended buffer row (the reference's own is a stale one); and against
+    the same sources as '.' ops with their pulses, on the GPU."""
+    ref = run_case('point_series', 'oracle', realsize)
+    out = run_case('point_series', 'hip', realsize)
+    pul = run_case('point_pulses', 'hip', realsize)
+    assert sorted(out) == sorted(ref) == sorted(pul)
+    bar = {4: 1e-4, 8: 1e-9}[realsize]
+    for k in ref:
+        assert rel_l2(out[k], ref[k]) <= bar, (k, rel_l2(out[k], ref[k]))
+        assert rel_l2(out[k], pul[k]) <= {4: 1e-5, 8: 1e-12}[realsize], (k, rel_l2(out[k], pul[k]))
