@@ -528,7 +528,7 @@ __global__ void k_boxop(T *f, View g, Box b, int op, T val, const T *src) {
 template <class T>
 __global__ void k_plane_copy(T *f, View g, int n0, int n1, int n2, int ax, int dst, int src) {
     const int n[3] = {n0, n1, n2};
-    const int b = (ax + 1) % 3, c = (ax + 2) % 3;
+    const int b = ax == 0 ? 1 : 0, c = ax == 2 ? 1 : 2;  // in-plane axes, the lower (faster in memory) first
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < (long)n[b] * n[c]; q += (long)gridDim.x * blockDim.x) {
         int idx[3];
         idx[b] = 1 + q % n[b];
@@ -637,7 +637,7 @@ __global__ void k_maxnorm(const T *a0, const T *a1, const T *a2, const T *b0, co
 template <class T>
 __global__ void k_vbc_plane(Params<T> P, int ax, int dst, int src, int mask, T s0, T s1, T s2, int kind) {
     const Grid<T> &g = P.g;
-    const int b = (ax + 1) % 3, c = (ax + 2) % 3;
+    const int b = ax == 0 ? 1 : 0, c = ax == 2 ? 1 : 2;  // in-plane axes, the lower (faster in memory) first
     const int nb = g.nm[b], nc = g.nm[c];
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < (long)nb * nc; q += (long)gridDim.x * blockDim.x) {
         int idx[3];
@@ -667,7 +667,7 @@ template <class T>
 __global__ void k_rupture(Params<T> P, Fault<T> F, int it, T tm, int fused) {
     const Grid<T> &g = P.g;
     const int a = P.ifn - 1;
-    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;  // in-plane axes, the lower (faster in memory) first
     const int nb = g.nm[b], nc = g.nm[c];
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < (long)nb * nc; q += (long)gridDim.x * blockDim.x) {
         int idx[3];
@@ -799,7 +799,7 @@ template <class T>
 __global__ void k_fault_time(Params<T> P, Fault<T> F, T tm, T svtol) {
     const Grid<T> &g = P.g;
     const int a = P.ifn - 1;
-    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;  // in-plane axes, the lower (faster in memory) first
     const int nb = g.nm[b], nc = g.nm[c];
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < (long)nb * nc; q += (long)gridDim.x * blockDim.x) {
         int idx[3];
@@ -840,7 +840,7 @@ template <class T>
 __global__ void k_fault_svold(Params<T> P, Fault<T> F) {
     const Grid<T> &g = P.g;
     const int a = P.ifn - 1;
-    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;  // in-plane axes, the lower (faster in memory) first
     const int nb = g.nm[b], nc = g.nm[c];
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < (long)nb * nc; q += (long)gridDim.x * blockDim.x) {
         int idx[3];
@@ -867,7 +867,7 @@ template <class T>
 __global__ void k_fault_sa(Params<T> P, Fault<T> F) {
     const Grid<T> &g = P.g;
     const int a = P.ifn - 1;
-    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;  // in-plane axes, the lower (faster in memory) first
     const int nb = g.nm[b], nc = g.nm[c];
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < (long)nb * nc; q += (long)gridDim.x * blockDim.x) {
         int idx[3];
