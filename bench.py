@@ -32,8 +32,9 @@ boundary launch stamping a pair of HIP events the same way.
 batched_vs_plain: the batched ensemble (64 seeded scenarios, 48..160 nodes
 per edge, PML, FP32; batched_leg) against the same scenarios run one after
 another.  grad_over_forward: the adjoint gradient on 200^3 nodes (free
-surfaces, 8 double couples, 64 surface receivers, sqrt(N) checkpoints;
-adjoint_leg) against one plain forward run of the same model.
+surfaces, 8 double couples, 64 surface receivers; the forward's last states
+kept in device memory, sqrt(N) checkpoints before them; adjoint_leg)
+against one plain forward run of the same model.
 
 cpu_baseline: the reference solver itself (the amdflang -O3 -fopenmp build of
 the reference Fortran, oracle/_ref/r4/sord.x) on this machine's host cores, on
@@ -101,6 +102,8 @@ def parse():
     p.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient leg')
     p.add_argument('--adjoint-n', type=int, default=200, help='adjoint mesh edge (nodes)')
     p.add_argument('--adjoint-steps', type=int, default=1500)
+    p.add_argument('--adjoint-resident', type=int, default=-1,
+                   help='forward states kept in device memory (-1: as many as fit, 0: checkpoints only)')
     p.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                    help='only copy MIB MiB device to device (calibration of the HBM traffic counters)')
     return p.parse_args()
@@ -247,7 +250,7 @@ def batched_leg(nscen, steps, warmup=5, tune_steps=10):
     }
 
 
-def adjoint_leg(n, nt):
+def adjoint_leg(n, nt, resident=-1):
     """Gradient of the waveform misfit against one plain forward run.
 
     Workload (coseis_amd.configs.adjoint_200): n^3 nodes, dx = 50 m, nt
@@ -257,12 +260,15 @@ def adjoint_leg(n, nt):
     HIP events (sord_bench).  Gradient: the checkpointed forward (every
     round(sqrt(nt)) steps, traces gathered each step) plus the backward sweep
     (recomputation of each segment, adjoint steps), both HIP-event timed
-    inside the library.  Initialisation is excluded from both."""
+    inside the library.  Initialisation is excluded from both.  resident=-1:
+    the forward's last states stay in device memory as far as 90 % of the
+    free memory allows, and only the segments before them are checkpointed
+    and recomputed (sord_adjoint_set_resident)."""
     import numpy as np
     import torch
     from coseis_amd import adjoint, configs
     w = configs.adjoint_200(n=n, nt=nt)
-    prob = adjoint.Problem(w['prm'], w['receivers'], realsize=4)
+    prob = adjoint.Problem(w['prm'], w['receivers'], realsize=4, resident=resident)
     obs = adjoint.traces(*w['observed'], prob)
     s, d = prob.open(*w['model'])
     cwd = os.getcwd()
@@ -280,6 +286,7 @@ def adjoint_leg(n, nt):
     J.backward()
     wall = time.perf_counter() - t0
     ms_f, ms_b = prob.last_timing
+    first, res_bytes = prob.last_resident
     g = [x.grad.numpy() for x in m]
     return round((ms_f + ms_b) / ms_plain, 3), {
         'nodes': n ** 3, 'steps': nt, 'checkpoint_every': prob.every, 'receivers': len(w['receivers']),
@@ -287,6 +294,7 @@ def adjoint_leg(n, nt):
         'backward_ms': round(ms_b, 2), 'gradient_wall_s': round(wall, 3),
         'misfit': float(J.detach()), 'grad_norm': [float(np.linalg.norm(x)) for x in g],
         'plain_gnode_s': round(n ** 3 * nt / ms_plain / 1e6, 3),
+        'resident_from_step': first, 'resident_gb': round(res_bytes / 1e9, 1),
     }
 
 
@@ -385,7 +393,7 @@ def main():
         line['batched_vs_plain'], line['batched'] = batched_leg(a.batch_scenarios, a.batch_steps)
         progress('batched leg: %s x plain' % line['batched_vs_plain'])
     if not a.no_adjoint:
-        line['grad_over_forward'], line['adjoint'] = adjoint_leg(a.adjoint_n, a.adjoint_steps)
+        line['grad_over_forward'], line['adjoint'] = adjoint_leg(a.adjoint_n, a.adjoint_steps, a.adjoint_resident)
         progress('adjoint leg: gradient / forward %s' % line['grad_over_forward'])
     if not a.no_cpu:
         line['cpu_baseline'] = cpu_baseline(prm, files, a.cpu_steps)

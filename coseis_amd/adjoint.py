@@ -5,8 +5,9 @@ Gradient of a waveform misfit with respect to per-cell rho, vp, vs.
 as cell input files, runs the solver with checkpoints every `every` steps
 (sqrt(nt) by default) and returns the velocity traces at the receivers; its
 backward runs the hand-written adjoint step kernels (sord_adjoint.hip) over
-the recomputed segments and returns dJ/drho, dJ/dvp, dJ/dvs for the incoming
-dJ/dtraces.  The mesh must be rectilinear with free-surface faces, no fault;
+the recomputed segments (or, with Problem(resident=-1), over the forward's
+last states kept in device memory) and returns dJ/drho, dJ/dvp, dJ/dvs for
+the incoming dJ/dtraces.  The mesh must be rectilinear with free-surface faces, no fault;
 sources are moment tensors (field-I/O ops on mxx..mxy).
 
     prob = Problem(prm, receivers)          # prm: cst.sord parameters without rho/vp/vs
@@ -28,7 +29,7 @@ from . import solver, sord
 class Problem(object):
     """Mesh, sources and receivers of a gradient computation."""
 
-    def __init__(self, prm, receivers, every=None, realsize=4, device=0):
+    def __init__(self, prm, receivers, every=None, realsize=4, device=0, resident=0):
         self.prm = dict(prm)
         for f in 'rho', 'vp', 'vs':
             self.prm.pop(f, None)
@@ -37,8 +38,12 @@ class Problem(object):
         self.every = every or max(1, int(round(math.sqrt(nt))))
         self.realsize = realsize
         self.device = device
+        # forward states kept in device memory instead of recomputed (-1: as
+        # many as fit in the free memory)
+        self.resident = resident
         self.cells = tuple(n - 1 for n in self.prm['shape'][:3])
         self.last_timing = None
+        self.last_resident = None  # (first resident state, bytes) of the last backward
 
     def open(self, rho, vp, vs):
         """Stage the run in a scratch directory; returns (solver, directory)."""
@@ -59,7 +64,7 @@ class Problem(object):
             # the scratch directory goes with the solver (a graph whose backward
             # never runs frees both when it is dropped)
             weakref.finalize(s, shutil.rmtree, d, True)
-            s.adjoint_setup(self.receivers, self.every)
+            s.adjoint_setup(self.receivers, self.every, self.resident)
         except Exception:
             os.chdir(cwd)
             shutil.rmtree(d)
@@ -91,6 +96,7 @@ class SordGradient(torch.autograd.Function):
             os.chdir(d)
             g = s.adjoint_backward(grad.detach().cpu().numpy(), problem.cells)
             problem.last_timing = s.adjoint_timing()
+            problem.last_resident = s.adjoint_resident()
         finally:
             os.chdir(cwd)
             s.close()

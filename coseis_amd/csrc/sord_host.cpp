@@ -1242,6 +1242,20 @@ extern "C" int sord_adjoint_setup(void *h, int nrec, const int32_t *rec_nodes, i
     });
 }
 
+extern "C" int sord_adjoint_set_resident(void *h, int64_t states) {
+    return dispatch(h, [&](auto &S) {
+        if (S.adj.on) fail(SORD_ESTATE, "sord_adjoint_set_resident after sord_adjoint_setup");
+        S.adj.resident_req = states < 0 ? -1 : (long)states;
+    });
+}
+
+extern "C" int sord_adjoint_resident(void *h, int64_t *first_state, int64_t *bytes) {
+    return dispatch(h, [&](auto &S) {
+        *first_state = S.adj.t0;
+        *bytes = (int64_t)S.adj.tail_bytes;
+    });
+}
+
 extern "C" int sord_adjoint_forward(void *h, void *traces) {
     return dispatch(h, [&](auto &S) {
         using T = typename std::remove_reference<decltype(S)>::type::real_t;

@@ -22,6 +22,7 @@ EXPORTS = [
     'sord_batch_create', 'sord_batch_set_graph', 'sord_batch_set_streams', 'sord_batch_set_tiles', 'sord_batch_init', 'sord_batch_step', 'sord_batch_run',
     'sord_batch_bench', 'sord_batch_size', 'sord_batch_destroy',
     'sord_adjoint_setup', 'sord_adjoint_forward', 'sord_adjoint_backward', 'sord_adjoint_timing',
+    'sord_adjoint_set_resident', 'sord_adjoint_resident',
     'sord_last_error', 'sord_destroy',
 ]
 
@@ -85,6 +86,8 @@ def load():
     lib.sord_adjoint_forward.argtypes = [vp, vp]
     lib.sord_adjoint_backward.argtypes = [vp, vp, vp, vp, vp]
     lib.sord_adjoint_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.sord_adjoint_set_resident.argtypes = [vp, ctypes.c_int64]
+    lib.sord_adjoint_resident.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int
     lib.sord_last_error.restype = ctypes.c_char_p
@@ -193,9 +196,12 @@ class Solver(object):
         return [(m[i], ms[i], nodes[i], damped[i]) for i in range(min(n.value, cap))]
 
     # ---- adjoint gradient (sord_adjoint_*) ----
-    def adjoint_setup(self, receivers, every):
-        """Receivers: (n, 3) 0-based node indices; checkpoint every `every` steps."""
+    def adjoint_setup(self, receivers, every, resident=0):
+        """Receivers: (n, 3) 0-based node indices; checkpoint every `every`
+        steps; keep the last `resident` forward states in device memory (-1:
+        as many as fit; sord_adjoint_set_resident)."""
         import numpy as np
+        _check(_lib.sord_adjoint_set_resident(self._h, int(resident)))
         rec = np.ascontiguousarray(receivers, dtype=np.int32)
         self._nrec = len(rec)
         _check(_lib.sord_adjoint_setup(self._h, self._nrec, rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
@@ -220,6 +226,12 @@ class Solver(object):
         _check(_lib.sord_adjoint_backward(self._h, res.ctypes.data, g[0].ctypes.data, g[1].ctypes.data,
                                           g[2].ctypes.data))
         return [x.reshape(cells[::-1]).T for x in g]
+
+    def adjoint_resident(self):
+        """(first resident state, bytes) of the adjoint's resident tail."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib.sord_adjoint_resident(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def adjoint_timing(self):
         """Wall time (ms) of the last adjoint forward and backward."""

@@ -203,6 +203,17 @@ void sord_batch_destroy(void *batch);
  * the state every checkpoint_every steps (about sqrt(nt) is the usual
  * choice: memory and recomputation both grow as nt/every + every). */
 int sord_adjoint_setup(void *handle, int nrec, const int32_t *rec_nodes, int checkpoint_every);
+/* Before sord_adjoint_setup: how many of the forward run's last states to
+ * keep resident in device memory, so that the backward sweep reads them
+ * instead of recomputing their segments: -1 as many as fit in 90 % of the
+ * free memory (less 2 GB), 0 none (the default: checkpoints only), n > 0 at
+ * most n.  Rounded to whole segments.  No counterpart in the reference. */
+int sord_adjoint_set_resident(void *handle, int64_t states);
+
+/* After sord_adjoint_setup: the first resident state (1-based step; 0 when
+ * none is kept) and the bytes they take. */
+int sord_adjoint_resident(void *handle, int64_t *first_state, int64_t *bytes);
+
 /* Run the remaining steps, storing checkpoints; traces (host, realsize
  * floats) receives [nt][nrec][3]: the velocity at each receiver at the
  * start of each step. */

@@ -90,6 +90,25 @@ def test_gradient_does_not_depend_on_checkpoints():
             assert np.abs(g[i] - base[i]).max() <= 1e-12 * np.abs(base[i]).max(), (every, i)
 
 
+def test_gradient_does_not_depend_on_resident_states():
+    """The forward's last states kept in device memory (the sweep reads them
+    instead of recomputing their segments) give the gradient of the
+    checkpointed run: none, some segments, all of them."""
+    prob, mat, w = problem((10, 9, 8), 30, 8, every=4)
+    base = gradient(prob, mat, w)
+    for resident in (13, 40, -1):
+        prob.resident = resident
+        s, _ = prob.open(*mat)
+        first, nbytes = s.adjoint_resident()
+        s.close()
+        print('resident', resident, 'first state', first, 'bytes', nbytes, flush=True)
+        assert first > 0 and nbytes > 0
+        assert (first == 1) == (resident != 13), first
+        g = gradient(prob, mat, w)
+        for i in range(3):
+            assert np.abs(g[i] - base[i]).max() <= 1e-12 * np.abs(base[i]).max(), (resident, i)
+
+
 def test_fp32_gradient_matches_fp64():
     """The FP32 gradient on a 48^3 mesh against the FP64 gradient of the same
     problem: relative L2 of each field within 1e-3."""
