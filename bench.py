@@ -261,7 +261,7 @@ def adjoint_leg(n, nt, resident=-1):
     round(sqrt(nt)) steps, traces gathered each step) plus the backward sweep
     (recomputation of each segment, adjoint steps), both HIP-event timed
     inside the library.  Initialisation is excluded from both.  resident=-1:
-    the forward's last states stay in device memory as far as 90 % of the
+    the forward's last states stay in device memory as far as 95 % of the
     free memory allows, and only the segments before them are checkpointed
     and recomputed (sord_adjoint_set_resident)."""
     import numpy as np

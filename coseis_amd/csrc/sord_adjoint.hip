@@ -304,7 +304,40 @@ void launch_rec_gather(T *const v[3], const Grid<T> &g, const int *rec, int n, T
     if (n > 0) hipLaunchKernelGGL((k_rec_gather<T>), dim3((n + 63) / 64), dim3(64), 0, s, v[0], v[1], v[2], g, rec, n, out);
 }
 
+// dJ/drho, dJ/dvp, dJ/dvs of every real cell from the material sums: mu =
+// rho vs^2, lambda = rho vp^2 - 2 mu, yy through lambda and mu, 1/m_n = 8 /
+// (sum of the 8 cells' rho V) (one thread per cell, the host restatement's
+// order of operations)
+template <class T>
+__global__ void k_adj_chain(AdjChain<T> c) {
+    const int n0 = c.hi[0] - c.lo[0] + 1, n1 = c.hi[1] - c.lo[1] + 1, n2 = c.hi[2] - c.lo[2] + 1;
+    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (q >= (long)n0 * n1 * n2) return;
+    const int j = c.lo[0] + (int)(q % n0), k = c.lo[1] + (int)((q / n0) % n1), l = c.lo[2] + (int)(q / ((long)n0 * n1));
+    const long o = c.g.at(j, k, l);
+    const T rho = c.rho[o], vp = c.vp[o], vs = c.vs[o];
+    const T L = c.lam[o], M = c.mu[o], D = L + T(2) * M;
+    const T dyl = D == 0 ? T(0) : c.sy * M * M / (D * D);
+    const T dym = D == 0 ? T(0) : c.sy * (L * L + T(2) * L * M + T(2) * M * M) / (D * D);
+    const T jl = c.gl[o] + c.gy[o] * dyl, jm = c.gmu[o] + c.gy[o] * dym;
+    T jr = T(0);  // through the node masses of the 8 corners
+    for (int kq = 0; kq < 8; kq++) {
+        const long on = c.g.at(j + (kq & 1), k + (kq >> 1 & 1), l + (kq >> 2 & 1));
+        const T m = c.mr[on];
+        jr += c.gm[on] * (-m * m * c.vol / T(8));
+    }
+    c.grho[q] = jr + jl * (vp * vp - T(2) * vs * vs) + jm * vs * vs;
+    c.gvp[q] = jl * T(2) * rho * vp;
+    c.gvs[q] = jl * (-T(4) * rho * vs) + jm * T(2) * rho * vs;
+}
+template <class T>
+void launch_adj_chain(const AdjChain<T> &c, hipStream_t s) {
+    const long n = (long)(c.hi[0] - c.lo[0] + 1) * (c.hi[1] - c.lo[1] + 1) * (c.hi[2] - c.lo[2] + 1);
+    if (n > 0) hipLaunchKernelGGL((k_adj_chain<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c);
+}
+
 #define INSTA(T)                                                                                       \
+    template void launch_adj_chain<T>(const AdjChain<T> &, hipStream_t);                              \
     template void launch_adjoint<T>(const AdjP<T> &, const Work *, int, hipStream_t);                 \
     template void launch_adj_inject<T>(T *const *, const Grid<T> &, const int *, int, const T *, hipStream_t); \
     template void launch_rec_gather<T>(T *const *, const Grid<T> &, const int *, int, T *, hipStream_t);

@@ -193,6 +193,18 @@ struct AdjP {
     int nlo[3], nhi[3];        // real nodes
 };
 
+// the chain rule of the adjoint's material sums to cell rho, vp, vs
+template <class T>
+struct AdjChain {
+    Grid<T> g;
+    const T *gl, *gmu, *gy, *gm;     // sums: cells lambda, mu, yy; nodes 1/mass
+    const T *rho, *vp, *vs;          // cell rho, vp, vs after the limits
+    const T *lam, *mu, *mr;          // cell lambda, mu; node 1/mass
+    T *grho, *gvp, *gvs;             // out: real cells, Fortran order
+    T vol, sy;                       // cell volume; hourglass scale
+    int lo[3], hi[3];                // real cells
+};
+
 // statistics slots of one scenario of a batch to clear before a step
 template <class T>
 struct StatZero {

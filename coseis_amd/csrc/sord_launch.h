@@ -46,6 +46,7 @@ template <class T> void launch_fused(const Params<T> &P, const PointOps<T> &ops,
 template <class T> int boundary_blocks_per_cu();
 template <class T> FusedC<T> fused_consts(const Params<T> &P, const T sp[3], const Work *work);
 template <class T> void launch_adjoint(const AdjP<T> &A, const Work *work, int nwork, hipStream_t s);
+template <class T> void launch_adj_chain(const AdjChain<T> &c, hipStream_t s);
 template <class T> void launch_adj_inject(T *const vh[3], const Grid<T> &g, const int *rec, int n, const T *res, hipStream_t s);
 template <class T> void launch_rec_gather(T *const v[3], const Grid<T> &g, const int *rec, int n, T *out, hipStream_t s);
 template <class T> void launch_zero_stats(const StatZero<T> *z, int n, hipStream_t s);
