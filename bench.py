@@ -99,6 +99,8 @@ def parse():
     p.add_argument('--no-batch', action='store_true', help='skip the batched-ensemble comparison')
     p.add_argument('--batch-scenarios', type=int, default=64)
     p.add_argument('--batch-steps', type=int, default=100)
+    p.add_argument('--batch-direct', action='store_true',
+                   help='launch the batch steps directly instead of replaying HIP graphs')
     p.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient leg')
     p.add_argument('--adjoint-n', type=int, default=200, help='adjoint mesh edge (nodes)')
     p.add_argument('--adjoint-steps', type=int, default=1500)
@@ -195,16 +197,18 @@ def cpu_baseline(prm, files, steps):
     }
 
 
-def batched_leg(nscen, steps, warmup=5, tune_steps=10):
+def batched_leg(nscen, steps, warmup=5, tune_steps=10, graph=True):
     """Batched ensemble against sequential plain runs of the same scenarios.
 
     The ensemble (coseis_amd.configs.ensemble): nscen seeded scenarios with
     edges log-uniform in 48..160 nodes, dx = 100 m, dt = 0.008 s, 1-4 layers,
     a double couple each, 10-node PML, FP32.  Batched: one solver.Batch; after
     warmup steps the autotuner picks the planes per grid-size bucket (its
-    trials advance the batch), then `steps` steps are timed (HIP events).
-    Plain: each scenario in its own solver.Solver, warmup steps, then the same
-    `steps` timed; the times add up.  Both exclude initialisation."""
+    trials advance the batch), then `steps` steps are timed (HIP events),
+    each step's merged launches replayed from HIP graphs (graph=False:
+    launched directly).  Plain: each scenario in its own solver.Solver, warmup
+    steps, then the same `steps` timed; the times add up.  Both exclude
+    initialisation."""
     from coseis_amd import autotune, configs, solver, sord
     ens = configs.ensemble(n=nscen, nt=warmup + tune_steps * 14 + steps + 8)
     shapes = [p['shape'] for p, _ in ens]
@@ -225,6 +229,7 @@ def batched_leg(nscen, steps, warmup=5, tune_steps=10):
             init_s = time.perf_counter() - t0
             b.step(warmup)
             choice, _, trials = autotune.tune_batch(b, shapes, steps=tune_steps)
+            graphs = b.set_graph(graph)
             ms_b, nu_b = b.bench(steps)
         ms_p, nu_p = 0.0, 0
         for q, text in enumerate(texts):
@@ -246,7 +251,7 @@ def batched_leg(nscen, steps, warmup=5, tune_steps=10):
         'batched_gnode_s': round(rb, 3), 'batched_ms_per_step': round(ms_b / steps, 4),
         'plain_sequential_gnode_s': round(rp, 3), 'plain_ms_per_step_sum': round(ms_p / steps, 4),
         'tiles_by_bucket': {str(k): list(v) for k, v in sorted(choice.items())},
-        'autotune_trials': len(trials), 'init_s': round(init_s, 2),
+        'autotune_trials': len(trials), 'init_s': round(init_s, 2), 'graphs': bool(graphs),
     }
 
 
@@ -390,7 +395,7 @@ def main():
         line['roofline']['boundary'] = boundary_rooflines(bnd, a.realsize, a.boundary_steps)
     progress('plain forward: %.3f Gnode/s' % value)
     if not a.no_batch:
-        line['batched_vs_plain'], line['batched'] = batched_leg(a.batch_scenarios, a.batch_steps)
+        line['batched_vs_plain'], line['batched'] = batched_leg(a.batch_scenarios, a.batch_steps, graph=not a.batch_direct)
         progress('batched leg: %s x plain' % line['batched_vs_plain'])
     if not a.no_adjoint:
         line['grad_over_forward'], line['adjoint'] = adjoint_leg(a.adjoint_n, a.adjoint_steps, a.adjoint_resident)

@@ -308,8 +308,9 @@ class Batch(object):
         _check(lib.sord_batch_init(self._h))
 
     def set_graph(self, on=True):
-        """Replay each step's merged launches from HIP graphs (on) or launch
-        them directly (off, the default); returns whether graphs are in use."""
+        """Replay each step's merged launches from HIP graphs (on, the
+        default) or launch them directly (off); None only queries; returns
+        whether graphs are in use."""
         u = ctypes.c_int()
         _check(_lib.sord_batch_set_graph(self._h, -1 if on is None else int(bool(on)), ctypes.byref(u)))
         return bool(u.value)

@@ -174,9 +174,9 @@ int sord_batch_create(int n, const char *const *sord_in_texts, const char *const
  * only (the autotuner's hook). */
 int sord_batch_set_tiles(void *batch, int index, int planes, int planes_boundary);
 int sord_batch_init(void *batch);
-/* on = 1: each step's merged launches (scenario table copy, boundary and
- * interior launches, stream fork and join) are captured once per slot of the
- * table ring into a HIP graph and replayed; 0: launched directly (default);
+/* on = 1 (default): each step's merged launches (scenario table copy,
+ * boundary and interior launches, stream fork and join) are captured once per
+ * slot of the table ring into a HIP graph and replayed; 0: launched directly;
  * -1: query only.  *in_use (if not NULL) receives 1 while graphs are in use (a
  * capture that fails switches them off).  Results are the same either way. */
 int sord_batch_set_graph(void *batch, int on, int *in_use);

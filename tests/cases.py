@@ -656,14 +656,13 @@ def w_layered_321(realsize=4):
 
 
 def w_rupture_400(realsize=4):
-    """Spontaneous rupture, 400 x 200 x 200 nodes (1500 steps in FP32, 400 in
-    FP64, by which time the front has left the nucleation patch by about
-    6 km), on every 4th x and every 2nd z node of the fault plane: slip,
-    peak slip rate, shear traction, rupture and arrest time at the last
-    step, slip rate at step 250; velocity at three off-fault receivers every
-    step."""
+    """Spontaneous rupture, 400 x 200 x 200 nodes, 1500 steps (12 s: the
+    front crosses the fault and arrests), on every 4th x and every 2nd z node
+    of the fault plane: slip, peak slip rate, shear traction, rupture and
+    arrest time at the last step, slip rate at step 250; velocity at three
+    off-fault receivers every step."""
     from coseis_amd import configs
-    nt = 1500 if realsize == 4 else 400
+    nt = 1500
     prm, files = configs.rupture_400(nt=nt)
     for f in ['sux', 'suy', 'suz', 'svm', 'tsm', 'trup']:
         prm.pop(f, None)

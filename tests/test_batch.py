@@ -65,8 +65,7 @@ def test_batch_matches_separate_runs(graph):
             texts.append(text)
             dirs.append(d)
         with solver.Batch(texts, dirs, realsize=realsize) as b:
-            if graph:
-                assert b.set_graph(True)
+            assert b.set_graph(graph) == graph
             b.run()
             assert b.set_graph(None) == graph  # the captures succeeded
         for c, ds, db in zip(NAMES, single, dirs):

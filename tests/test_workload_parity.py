@@ -3,8 +3,8 @@
 Fixtures (tests/golden/w_*_r{4,8}.npz) are the reference solver's own output
 files for the cases WORKLOADS of tests/cases.py: the explosion (61^3, 200
 steps), the layered half-space (321 x 321 x 161; 1500 steps in FP32, 300 in
-FP64), the spontaneous rupture (400 x 200 x 200; 1500 steps in FP32, 400 in
-FP64), the adjoint workload's forward run on its 48^3 subgrid (1500 steps)
+FP64), the spontaneous rupture (400 x 200 x 200, 1500 steps), the adjoint
+workload's forward run on its 48^3 subgrid (1500 steps)
 and four scenarios of the batched ensemble (800 steps, run here as one
 batch).  Bars (tests/conftest.py check_outputs): relative L2 of every output
 (vectors as a whole) <= 1e-4 in FP32 and <= 1e-9 in FP64; rupture time and
