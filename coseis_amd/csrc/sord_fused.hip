@@ -520,17 +520,38 @@ void k_fused_batch_stat(const Scen<T> *__restrict__ sc, const Work *__restrict__
 // Rows: an item of width je-jb+1 runs LW = width+2 lanes per node row (Work.pad0)
 // and R = 64/LW rows per wavefront, so narrow items (the x slabs) fill the
 // wavefront; node rows of the block are w*R + lane/LW, with halo rows 0 and W*R-1.
-// LDS of the boundary kernel: 7 node values and 21 cell coefficients per slot,
+// LDS of the boundary kernel: 12 node values and 21 cell coefficients per slot,
 // W * 64 + 64 slots (lanes past R*LW use the last 64)
 template <int W>
-constexpr int boundary_lds_slots() { return 28 * (W * 64 + 64); }
+constexpr int boundary_lds_slots() { return 33 * (W * 64 + 64); }
+
+// Sums over the four nodes (j, k), (j+1, k), (j, k+1), (j+1, k+1) of one node
+// plane, from which a cell's differences and hourglass modes follow: each
+// plane's sums serve the cell below it and the cell above it.  With f_q the
+// node values (q = dj + 2 dk):
+//   J = (f1 - f0) + (f3 - f2)   K = (f2 - f0) + (f3 - f1)
+//   S = f0 + f1 + f2 + f3       X = f0 - f1 - f2 + f3
+// A cell between planes lo and hi has the differences along j, k, l
+// J_lo + J_hi, K_lo + K_hi, S_hi - S_lo, and the hourglass modes
+// (s_k s_l, s_j s_l, s_j s_k, s_j s_k s_l) K_hi - K_lo, J_hi - J_lo,
+// X_lo + X_hi, X_lo - X_hi (s = +1 at the lower corner).
+template <class T>
+struct PlaneSums {
+    T w[3][3];  // u + gamma v: J, K, S (strain)
+    T F[3][3];  // h1 u + dt h2 v: J, K, X (full hourglass)
+    T v[3][4];  // v: J, K, S (strain in a damped direction), X (viscous hourglass)
+    T u[3][3];  // u: J, K, S (strain in the other directions of a PML slab)
+};
 
 template <class T, int W, int MASK, bool FS>
 __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
                                               const Work &wk, T *lds) {
     constexpr int NS = W * 64 + 64;
+    // node values exchanged per plane: u + gamma v and h1 u + dt h2 v, and with
+    // damping also v and u
+    constexpr int NF = MASK ? 12 : 6;
     T (*sN)[NS] = reinterpret_cast<T (*)[NS]>(lds);
-    T (*sC)[NS] = reinterpret_cast<T (*)[NS]>(lds + 7 * NS);
+    T (*sC)[NS] = reinterpret_cast<T (*)[NS]>(lds + NF * NS);
     const int lane = threadIdx.x, w = threadIdx.y;
     const int LW = wk.pad0, R = 64 / LW, NR = W * R;
     const int jj = lane % LW, rr = lane / LW;
@@ -542,7 +563,11 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     // one damped axis PA: its PML memory comes with the plane's other loads
     constexpr bool PF = MASK == 1 || MASK == 2 || MASK == 4;
     constexpr int PA = MASK == 2 ? 1 : MASK == 4 ? 2 : 0;
-    constexpr int NMAT = PF ? 14 : 4;
+    // per plane: lambda, mu, yy, 1/m; g and p of the damped axis (PF); their
+    // coefficients when the damped axis is l (along j and k they are constant
+    // over the lane's planes)
+    constexpr int NMAT = !PF ? 4 : PA == 2 ? 14 : 10;
+    constexpr int ZC = NMAT > 10 ? 10 : 0;  // the coefficients along l (unused otherwise)
     const int j = wk.jb - 1 + jj, k = wk.kb - 1 + row;
     const bool ownjk = valid && jj >= 1 && jj <= LW - 2 && row >= 1 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
     const bool cellv = valid && jj <= LW - 2 && row <= NR - 2 && j <= wk.je && k <= wk.ke;
@@ -556,6 +581,106 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     const int fa = P.ifn - 1;  // fault normal axis (-1: none)
     T uo[3], vo[3];
 
+    // ---- the lane's geometry along j and k, constant over its planes ----
+    // cell (j, k, .) inside the mesh and not the fault's zero-width cell; node
+    // (j, k, .) inside; node on a fault plane
+    const bool cjk = j >= P.i1cell[0] && j <= P.i2cell[0] && k >= P.i1cell[1] && k <= P.i2cell[1] &&
+                     !(fa == 0 && j == P.irup) && !(fa == 1 && k == P.irup);
+    const bool njk = ownjk && j >= P.i1node[0] && j <= P.i2node[0] && k >= P.i1node[1] && k <= P.i2node[1];
+    const bool fjk = (fa == 0 && (j == P.irup || j == P.irup + 1)) || (fa == 1 && (k == P.irup || k == P.irup + 1));
+    // force components that the boundary conditions of a node plane zero
+    // (sord/boundary_cond.f90:44-66; -1: the two tangential, 1: the normal,
+    // 3: all)
+    auto bc_zero = [&](int a, int n) -> int {
+        int bt = 0;
+        if (n == P.i1bc[a]) bt = P.bc1[a] % 10;
+        if (n == P.i2bc[a]) bt = P.bc2[a] % 10;
+        return bt == -1 ? (7 & ~(1 << a)) : bt == 1 ? 1 << a : bt == 3 ? 7 : 0;
+    };
+    const int zjk = bc_zero(0, j) | bc_zero(1, k);
+    // slab membership along the damped axes j, k: cell (g memory, source
+    // fields), node (p memory), node inside the undamped range (full hourglass)
+    bool cin[3] = {false, false, false}, nin[3] = {false, false, false};
+    bool hint_jk = true;
+    // PML memory of the lane's cell and node along j and k: pointers to plane 1,
+    // per-plane steps and component strides, coefficients
+    gptr<T> lg[2] = {nullptr, nullptr}, lgn[2] = {nullptr, nullptr}, lp[2] = {nullptr, nullptr};
+    int lgl[2] = {0, 0}, lgs[2] = {0, 0}, lpl[2] = {0, 0}, lps[2] = {0, 0};
+    T lc2[2] = {T(0), T(0)}, lc1[2] = {T(0), T(0)}, ln2[2] = {T(0), T(0)}, ln1[2] = {T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        if (!(MASK >> a & 1)) continue;
+        const int c = a ? k : j;
+        const bool clo = c <= P.i1pml[a];
+        cin[a] = clo || c >= P.i2pml[a] - 1;
+        const bool nlo = c <= P.i1pml[a];
+        nin[a] = nlo || c >= P.i2pml[a];
+        if (c < P.i1pml[a] + 1 || c > P.i2pml[a] - 1) hint_jk = false;
+        // memory layout (sord/arrays.f90): the slab index along a fastest (a
+        // = 0) or the middle index (a = 1)
+        if (cin[a] && cjk) {
+            const int si = clo ? c - P.i1cell[a] + 1 : P.i2cell[a] - c + 1;
+            const int pidx = clo ? c + g.nnoff[a] : g.nn[a] - c - g.nnoff[a];
+            const int dep = clo ? uniform_int(P.gn[a][a]) : uniform_int(P.gn[a + 3][a]);
+            long e;
+            if (a == 0) e = (long)(si - 1) + (long)dep * (k - 1), lgl[a] = dep * g.nm[1];
+            else e = (long)(j - 1) + (long)g.nm[0] * (si - 1), lgl[a] = g.nm[0] * dep;
+            lgs[a] = lgl[a] * g.nm[2];
+            lg[a] = (clo ? uniform_ptr(P.gm[a]) : uniform_ptr(P.gm[a + 3])) + e;
+            lgn[a] = (clo ? uniform_ptr(P.gmn[a]) : uniform_ptr(P.gmn[a + 3])) + e;
+            lc2[a] = P.dc2[pidx - 1], lc1[a] = P.dc1[pidx - 1];
+        }
+        if (nin[a] && njk) {
+            const int si = nlo ? c - P.i1node[a] + 1 : P.i2node[a] - c + 1;
+            const int pidx = nlo ? c + g.nnoff[a] : g.nn[a] - c - g.nnoff[a] + 1;
+            const int dep = nlo ? uniform_int(P.pn[a][a]) : uniform_int(P.pn[a + 3][a]);
+            long e;
+            if (a == 0) e = (long)(si - 1) + (long)dep * (k - 1), lpl[a] = dep * g.nm[1];
+            else e = (long)(j - 1) + (long)g.nm[0] * (si - 1), lpl[a] = g.nm[0] * dep;
+            lps[a] = lpl[a] * g.nm[2];
+            lp[a] = (nlo ? uniform_ptr(P.p[a]) : uniform_ptr(P.p[a + 3])) + e;
+            ln2[a] = P.dn2[pidx - 1], ln1[a] = P.dn1[pidx - 1];
+        }
+    }
+    const long ejk = (long)(j - 1) + (long)g.nm[0] * (k - 1);  // (j, k) in a slab of axis l
+
+    // ---- per plane along l (uniform over the block) ----
+    // cell l inside the mesh and not the fault cell
+    auto creal_l = [&](int l) { return l >= P.i1cell[2] && l <= P.i2cell[2] && !(fa == 2 && l == P.irup); };
+    // PML memory of cell l (g) and node l (p) in the slab of axis l: element
+    // offset of (j, k), component stride, coefficient index, side
+    struct LSlot {
+        bool on, lo;
+        long e, s;
+        int pidx;
+    };
+    auto gslot_l = [&](int l) -> LSlot {
+        LSlot r{false, false, 0, 0, 0};
+        if (!(MASK & 4) || !creal_l(l)) return r;
+        r.lo = l <= P.i1pml[2];
+        if (!r.lo && l < P.i2pml[2] - 1) return r;
+        r.on = true;
+        const int si = r.lo ? l - P.i1cell[2] + 1 : P.i2cell[2] - l + 1;
+        r.pidx = r.lo ? l + g.nnoff[2] : g.nn[2] - l - g.nnoff[2];
+        const long dep = r.lo ? P.gn[2][2] : P.gn[5][2];
+        r.s = (long)g.nm[0] * g.nm[1] * dep;
+        r.e = ejk + (long)g.nm[0] * g.nm[1] * (si - 1);
+        return r;
+    };
+    auto pslot_l = [&](int l) -> LSlot {
+        LSlot r{false, false, 0, 0, 0};
+        if (!(MASK & 4) || l < P.i1node[2] || l > P.i2node[2]) return r;
+        r.lo = l <= P.i1pml[2];
+        if (!r.lo && l < P.i2pml[2]) return r;
+        r.on = true;
+        const int si = r.lo ? l - P.i1node[2] + 1 : P.i2node[2] - l + 1;
+        r.pidx = r.lo ? l + g.nnoff[2] : g.nn[2] - l - g.nnoff[2] + 1;
+        const long dep = r.lo ? P.pn[2][2] : P.pn[5][2];
+        r.s = (long)g.nm[0] * g.nm[1] * dep;
+        r.e = ejk + (long)g.nm[0] * g.nm[1] * (si - 1);
+        return r;
+    };
+
     // raw loads: node (u, v, gamma) of a plane; cell (lambda, mu, yy) and node 1/m
     auto fetch_node = [&](int l, T (&nd)[7]) {
         const long o = g.at(jc, kc, min(l, lmax));
@@ -563,100 +688,75 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
         for (int i = 0; i < 3; i++) nd[i] = P.u[i][o], nd[3 + i] = P.v[i][o];
         nd[6] = P.gam[o];
     };
-    // PML memory of the damped axis id: element index, component stride,
-    // coefficient row and side of cell (j, k, l) (g, stress stage) or node
-    // (j, k, l) (p, accel stage); false where it has none
-    auto g_slot = [&](int id, int l, long &e, long &stride, int &pidx, bool &lo) -> bool {
-        const int c3[3] = {j, k, l};
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (c3[a] < P.i1cell[a] || c3[a] > P.i2cell[a]) return false;
-        if (fa >= 0 && c3[fa] == P.irup) return false;
-        const int c = c3[id];
-        lo = c <= P.i1pml[id];
-        if (!lo && c < P.i2pml[id] - 1) return false;
-        const int si = lo ? c - P.i1cell[id] + 1 : P.i2cell[id] - c + 1;
-        pidx = lo ? c + g.nnoff[id] : g.nn[id] - c - g.nnoff[id];
-        const int dep = lo ? uniform_int(P.gn[id][id]) : uniform_int(P.gn[id + 3][id]);
-        if (id == 0) {
-            e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
-            stride = (long)dep * g.nm[1] * g.nm[2];
-        } else if (id == 1) {
-            e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
-            stride = (long)g.nm[0] * dep * g.nm[2];
-        } else {
-            e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
-            stride = (long)g.nm[0] * g.nm[1] * dep;
-        }
-        return true;
-    };
-    auto p_slot = [&](int id, int l, long &e, long &stride, int &pidx, bool &lo) -> bool {
-        const int n3[3] = {j, k, l};
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (n3[a] < P.i1node[a] || n3[a] > P.i2node[a]) return false;
-        const int n = n3[id];
-        lo = n <= P.i1pml[id];
-        if (!lo && n < P.i2pml[id]) return false;
-        const int si = lo ? n - P.i1node[id] + 1 : P.i2node[id] - n + 1;
-        pidx = lo ? n + g.nnoff[id] : g.nn[id] - n - g.nnoff[id] + 1;
-        const int dep = lo ? uniform_int(P.pn[id][id]) : uniform_int(P.pn[id + 3][id]);
-        if (id == 0) {
-            e = (long)(si - 1) + (long)dep * ((long)(k - 1) + (long)g.nm[1] * (l - 1));
-            stride = (long)dep * g.nm[1] * g.nm[2];
-        } else if (id == 1) {
-            e = (long)(j - 1) + (long)g.nm[0] * ((long)(si - 1) + (long)dep * (l - 1));
-            stride = (long)g.nm[0] * dep * g.nm[2];
-        } else {
-            e = (long)(j - 1) + (long)g.nm[0] * ((long)(k - 1) + (long)g.nm[1] * (si - 1));
-            stride = (long)g.nm[0] * g.nm[1] * dep;
-        }
-        return true;
-    };
     // lambda, mu, yy of cell l and 1/m of node l; with one damped axis (PF)
-    // also the PML memory of cell l (g, 3 components, then dc2, dc1) and of
-    // node l (p, then dn2, dn1), loaded a plane ahead like the rest: a load
-    // consumed at once makes the wave wait for every load issued before it,
-    // the next plane's prefetch included
+    // also the PML memory of cell l (g, 3 components) and node l (p, 3
+    // components), and along l their coefficients, loaded a plane ahead like
+    // the rest: a load consumed at once makes the wave wait for every load
+    // issued before it, the next plane's prefetch included
     auto fetch_cell = [&](int l, T (&m)[NMAT]) {
         const long o = g.at(jc, kc, min(l, lmax));
         m[0] = P.lam[o], m[1] = P.mu[o], m[2] = P.yy[o], m[3] = P.mr[o];
         if constexpr (PF) {
-            long e, stride;
-            int pidx;
-            bool lo;
 #pragma unroll
             for (int q = 4; q < NMAT; q++) m[q] = T(0);
-            if (g_slot(PA, l, e, stride, pidx, lo)) {
-                const auto gp = lo ? uniform_ptr(P.gm[PA]) : uniform_ptr(P.gm[PA + 3]);
+            if constexpr (PA < 2) {
+                if (cin[PA] && cjk && creal_l(l)) {
+                    const long e = (long)lgl[PA] * (l - 1);
 #pragma unroll
-                for (int ic = 0; ic < 3; ic++) m[4 + ic] = gp[e + ic * stride];
-                m[7] = P.dc2[pidx - 1], m[8] = P.dc1[pidx - 1];
-            }
-            if (ownjk && p_slot(PA, l, e, stride, pidx, lo)) {
-                const auto pp = lo ? uniform_ptr(P.p[PA]) : uniform_ptr(P.p[PA + 3]);
+                    for (int ic = 0; ic < 3; ic++) m[4 + ic] = lg[PA][e + (long)ic * lgs[PA]];
+                }
+                if (nin[PA] && njk && l >= P.i1node[2] && l <= P.i2node[2]) {
+                    const long e = (long)lpl[PA] * (l - 1);
 #pragma unroll
-                for (int ic = 0; ic < 3; ic++) m[9 + ic] = pp[e + ic * stride];
-                m[12] = P.dn2[pidx - 1], m[13] = P.dn1[pidx - 1];
+                    for (int ic = 0; ic < 3; ic++) m[7 + ic] = lp[PA][e + (long)ic * lps[PA]];
+                }
+            } else {
+                const LSlot gs = gslot_l(l);
+                if (gs.on && cjk) {
+                    const auto gp = gs.lo ? uniform_ptr(P.gm[2]) : uniform_ptr(P.gm[5]);
+#pragma unroll
+                    for (int ic = 0; ic < 3; ic++) m[4 + ic] = gp[gs.e + ic * gs.s];
+                    m[ZC] = P.dc2[gs.pidx - 1], m[ZC + 1] = P.dc1[gs.pidx - 1];
+                }
+                const LSlot ps = pslot_l(l);
+                if (ps.on && njk) {
+                    const auto pp = ps.lo ? uniform_ptr(P.p[2]) : uniform_ptr(P.p[5]);
+#pragma unroll
+                    for (int ic = 0; ic < 3; ic++) m[7 + ic] = pp[ps.e + ic * ps.s];
+                    m[ZC + 2] = P.dn2[ps.pidx - 1], m[ZC + 3] = P.dn1[ps.pidx - 1];
+                }
             }
         }
     };
-    // corner data of a plane: [corner dj + 2 dk][u 0..2, v 3..5, gamma 6]
-    auto exchange = [&](const T (&own)[7], T (&N)[4][7]) {
+    // the plane sums of a plane from the raw node loads (u 0..2, v 3..5,
+    // gamma 6): each node's values once, the row above through LDS, the next
+    // column through a lane shift of the row pair's sum and difference
+    auto exchange = [&](const T (&nd)[7], PlaneSums<T> &S) {
+        T own[NF];
 #pragma unroll
-        for (int q = 0; q < 7; q++) sN[q][t] = own[q];
+        for (int i = 0; i < 3; i++) {
+            own[i] = nd[i] + nd[6] * nd[3 + i];
+            own[3 + i] = C.h1 * nd[i] + C.dth2 * nd[3 + i];
+            if constexpr (MASK != 0) own[6 + i] = nd[3 + i], own[9 + i] = nd[i];
+        }
+#pragma unroll
+        for (int q = 0; q < NF; q++) sN[q][t] = own[q];
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 7; q++) {
+        for (int q = 0; q < NF; q++) {
             const T ab = sN[q][tab];
-            N[0][q] = own[q];
-            N[2][q] = ab;
-            N[1][q] = lane_next(own[q]);
-            N[3][q] = lane_next(ab);
+            const T e = own[q] + ab, d = ab - own[q];  // rows k + (k+1), (k+1) - k
+            const T ne = lane_next(e), nd1 = lane_next(d);
+            const T J = ne - e, K = d + nd1, Sm = e + ne, X = nd1 - d;
+            const int f = q / 3, c = q % 3;
+            if (f == 0) S.w[c][0] = J, S.w[c][1] = K, S.w[c][2] = Sm;
+            else if (f == 1) S.F[c][0] = J, S.F[c][1] = K, S.F[c][2] = X;
+            else if (f == 2) S.v[c][0] = J, S.v[c][1] = K, S.v[c][2] = Sm, S.v[c][3] = X;
+            else S.u[c][0] = J, S.u[c][1] = K, S.u[c][2] = Sm;
         }
     };
 
-    // cell (j, k, l) from the corner data of planes l (Nlo) and l+1 (Nhi):
+    // cell (j, k, l) from the plane sums of planes l (Slo) and l+1 (Shi):
     // coefficients per component at s_l = +1 (E) and -1 (Cc): [0..2] divergence
     // along x, y, z (with the sign of s_l on the z term), [3..5] full hourglass
     // (s_j, s_k, s_j s_k), [6..8] viscous hourglass.  wr: this block owns the
@@ -666,21 +766,12 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     // swaps the two after the launch).  st: the cell counts in the statistics
     // (not the spare plane past the item's last, whose damping this kernel's
     // mask may lack).
-    auto cell_at = [&](int l, const T (&mat)[NMAT], const T (&Nlo)[4][7], const T (&Nhi)[4][7], T (&E)[3][9],
+    auto cell_at = [&](int l, const T (&mat)[NMAT], const PlaneSums<T> &Slo, const PlaneSums<T> &Shi, T (&E)[3][9],
                        T (&Cc)[3][9], bool wr, bool st) {
-        const int c3[3] = {j, k, l};
-        bool real = true;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (c3[a] < P.i1cell[a] || c3[a] > P.i2cell[a]) real = false;
-        if (fa >= 0 && c3[fa] == P.irup) real = false;  // zero-width fault cell: 1/V = 0, yy = 0
+        const bool real = cjk && creal_l(l);
         const T lm = mat[0], m = mat[1], y = mat[2];
-        bool inLo[3], inS[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            inLo[a] = (MASK >> a & 1) && c3[a] <= P.i1pml[a];
-            inS[a] = inLo[a] || ((MASK >> a & 1) && c3[a] >= P.i2pml[a] - 1);
-        }
+        bool inS[3] = {cin[0], cin[1], false};
+        if (MASK & 4) inS[2] = l <= P.i1pml[2] || l >= P.i2pml[2] - 1;
         const T gsp[3] = {C.gx, C.gy, C.gz};
         T G[3][3];  // G[id][ic]
 #pragma unroll
@@ -688,49 +779,59 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
             bool other = false;
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                if (a != id && inS[a]) other = true;
-            const bool useV = inS[id], useU = !inS[id] && other;
+                if (a != id && (MASK >> a & 1) && inS[a]) other = true;
+            const bool useV = (MASK >> id & 1) && inS[id], useU = !useV && other;
 #pragma unroll
             for (int ic = 0; ic < 3; ic++) {
-                // corner values of the source field (v in the damped direction,
-                // u in other PML directions, u + gamma v elsewhere)
-                T f[8];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const T wl = Nlo[q][ic] + Nlo[q][6] * Nlo[q][3 + ic];
-                    const T wh = Nhi[q][ic] + Nhi[q][6] * Nhi[q][3 + ic];
-                    f[q] = useV ? Nlo[q][3 + ic] : useU ? Nlo[q][ic] : wl;
-                    f[4 + q] = useV ? Nhi[q][3 + ic] : useU ? Nhi[q][ic] : wh;
+                // difference of the source field: v in the damped direction, u
+                // in the other directions of a PML slab, u + gamma v elsewhere
+                auto diff = [&](const T (&lo)[3], const T (&hi)[3]) { return id == 2 ? hi[2] - lo[2] : lo[id] + hi[id]; };
+                const T dw = diff(Slo.w[ic], Shi.w[ic]);
+                T d = dw;
+                if constexpr (MASK != 0) {
+                    const T dv = id == 2 ? Shi.v[ic][2] - Slo.v[ic][2] : Slo.v[ic][id] + Shi.v[ic][id];
+                    const T du = diff(Slo.u[ic], Shi.u[ic]);
+                    d = useV ? dv : useU ? du : dw;
                 }
-                T d;
-                if (id == 0) d = ((f[1] - f[0]) + (f[3] - f[2])) + ((f[5] - f[4]) + (f[7] - f[6]));
-                else if (id == 1) d = ((f[2] - f[0]) + (f[3] - f[1])) + ((f[6] - f[4]) + (f[7] - f[5]));
-                else d = ((f[4] - f[0]) + (f[5] - f[1])) + ((f[6] - f[2]) + (f[7] - f[3]));
                 G[id][ic] = gsp[id] * d;
             }
             if ((MASK >> id & 1) && inS[id] && real) {
                 // split-field memory of the damped direction in the reference's
                 // units (volume-integrated strain), sord/stress.f90:42-131
-                long e, stride;
-                int pidx;
-                bool lo;
-                g_slot(id, l, e, stride, pidx, lo);  // true here: the cell is real and in the slab
-                const auto gp = lo ? uniform_ptr(P.gm[id]) : uniform_ptr(P.gm[id + 3]);
-                const auto gq = lo ? uniform_ptr(P.gmn[id]) : uniform_ptr(P.gmn[id + 3]);
                 T c2, c1, old[3];
-                if constexpr (PF) {
-                    c2 = mat[7], c1 = mat[8];
+                gptr<T> gq;
+                long e, s;
+                if (id < 2) {
+                    e = (long)lgl[id] * (l - 1), s = lgs[id];
+                    gq = lgn[id] + e;
+                    c2 = lc2[id], c1 = lc1[id];
+                    if constexpr (PF) {
 #pragma unroll
-                    for (int ic = 0; ic < 3; ic++) old[ic] = mat[4 + ic];
+                        for (int ic = 0; ic < 3; ic++) old[ic] = mat[4 + ic];
+                    } else {
+#pragma unroll
+                        for (int ic = 0; ic < 3; ic++) old[ic] = lg[id][e + ic * s];
+                    }
+                    e = 0;
                 } else {
-                    c2 = P.dc2[pidx - 1], c1 = P.dc1[pidx - 1];
+                    const LSlot gs = gslot_l(l);
+                    e = gs.e, s = gs.s;
+                    gq = gs.lo ? uniform_ptr(P.gmn[2]) : uniform_ptr(P.gmn[5]);
+                    if constexpr (PF) {
+                        c2 = mat[ZC], c1 = mat[ZC + 1];
 #pragma unroll
-                    for (int ic = 0; ic < 3; ic++) old[ic] = gp[e + ic * stride];
+                        for (int ic = 0; ic < 3; ic++) old[ic] = mat[4 + ic];
+                    } else {
+                        const auto gp = gs.lo ? uniform_ptr(P.gm[2]) : uniform_ptr(P.gm[5]);
+                        c2 = P.dc2[gs.pidx - 1], c1 = P.dc1[gs.pidx - 1];
+#pragma unroll
+                        for (int ic = 0; ic < 3; ic++) old[ic] = gp[e + ic * s];
+                    }
                 }
 #pragma unroll
                 for (int ic = 0; ic < 3; ic++) {
                     const T dv = c2 * (G[id][ic] * C.vol) + c1 * old[ic];
-                    if (wr) gq[e + ic * stride] = dv;
+                    if (wr) gq[e + ic * s] = dv;
                     G[id][ic] = dv * C.rvol;
                 }
             }
@@ -771,17 +872,18 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
         const T yf = real ? y : T(0), yv = real ? C.dth2 * y : T(0);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
+            // modes (s_k s_l, s_j s_l, s_j s_k, s_j s_k s_l) of h1 u + dt h2 v
+            // and, for the viscous hourglass of PML nodes, of v
             T h[4], hv[4];
-            T hl[4], hh[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                hl[q] = C.h1 * Nlo[q][c] + C.dth2 * Nlo[q][3 + c];
-                hh[q] = C.h1 * Nhi[q][c] + C.dth2 * Nhi[q][3 + c];
-            }
-            hg_modes<T>(hl[0], hl[1], hl[2], hl[3], hh[0], hh[1], hh[2], hh[3], h);
-            if (MASK) {  // viscous hourglass of PML nodes (none without damped cells)
-                hg_modes<T>(Nlo[0][3 + c], Nlo[1][3 + c], Nlo[2][3 + c], Nlo[3][3 + c], Nhi[0][3 + c],
-                            Nhi[1][3 + c], Nhi[2][3 + c], Nhi[3][3 + c], hv);
+            h[0] = Shi.F[c][1] - Slo.F[c][1];
+            h[1] = Shi.F[c][0] - Slo.F[c][0];
+            h[2] = Slo.F[c][2] + Shi.F[c][2];
+            h[3] = Slo.F[c][2] - Shi.F[c][2];
+            if constexpr (MASK != 0) {
+                hv[0] = Shi.v[c][1] - Slo.v[c][1];
+                hv[1] = Shi.v[c][0] - Slo.v[c][0];
+                hv[2] = Slo.v[c][3] + Shi.v[c][3];
+                hv[3] = Slo.v[c][3] - Shi.v[c][3];
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; q++) hv[q] = T(0);
@@ -804,29 +906,29 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     // The plane loop is unrolled by two with alternating register sets, as in
     // k_fused: no register holding a load in flight or a carried value is copied.
     T own[7], E[3][9];
-    T NA[4][7], NB[4][7];
+    PlaneSums<T> SA, SB;  // plane sums of two planes (lower, upper in turn)
     T CA[3][9], CB[3][9];
     T fA[7], mA[NMAT], fB[7], mB[NMAT];
     fetch_node(wk.lz0 - 1, fA);
-    exchange(fA, NA);
+    exchange(fA, SA);
     fetch_node(wk.lz0, fB);
     fetch_cell(wk.lz0 - 1, mB);
 #pragma unroll
     for (int i = 0; i < 3; i++) uo[i] = fB[i], vo[i] = fB[3 + i];
     __syncthreads();  // every wave has read the first plane from LDS
-    exchange(fB, NB);
+    exchange(fB, SB);
     fetch_node(wk.lz0 + 1, fA);
     fetch_cell(wk.lz0, mA);
-    cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, false, true);
+    cell_at(wk.lz0 - 1, mB, SA, SB, E, CA, false, true);
 
-    auto plane = [&](int l, bool act, const T (&cf)[7], const T (&cm)[NMAT], T (&nf)[7], T (&nm_)[NMAT], const T (&Nlo)[4][7],
-                     T (&Nhi)[4][7], const T (&carry)[3][9], T (&Cn)[3][9]) {
+    auto plane = [&](int l, bool act, const T (&cf)[7], const T (&cm)[NMAT], T (&nf)[7], T (&nm_)[NMAT],
+                     const PlaneSums<T> &Slo, PlaneSums<T> &Shi, const T (&carry)[3][9], T (&Cn)[3][9]) {
 #pragma unroll
         for (int q = 0; q < 7; q++) own[q] = cf[q];
         fetch_node(l + 2, nf);
         fetch_cell(l + 1, nm_);
-        exchange(own, Nhi);
-        cell_at(l, cm, Nlo, Nhi, E, Cn, ownjk && act, act);
+        exchange(own, Shi);
+        cell_at(l, cm, Slo, Shi, E, Cn, ownjk && act, act);
         // plane combination, then the k stage: own part (s_k = +1) and the part
         // sent to the row above (s_k = -1): divergence x, y, z, full (c0, cj), viscous (c0, cj)
         T R[3][7];
@@ -864,36 +966,47 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
             Hf[c] = (r[3] + r[4]) + lane_prev(r[3] - r[4]);
             Hv[c] = (r[5] + r[6]) + lane_prev(r[5] - r[6]);
         }
-        const int n3[3] = {j, k, l};
-        bool own_ = ownjk && act;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (n3[a] < P.i1node[a] || n3[a] > P.i2node[a]) own_ = false;
-        if (own_) {
+        const bool nl_in = l >= P.i1node[2] && l <= P.i2node[2];
+        if (njk && act && nl_in) {
             const long o = g.at(j, k, l);
             // PML momentum memory of each damped direction (sord/acceleration.f90:37-104)
 #pragma unroll
             for (int id = 0; id < 3; id++) {
                 if (!(MASK >> id & 1)) continue;
-                long e, stride;
-                int pidx;
-                bool lo;
-                if (!p_slot(id, l, e, stride, pidx, lo)) continue;  // not a PML node of axis id
-                const auto pp = lo ? uniform_ptr(P.p[id]) : uniform_ptr(P.p[id + 3]);
                 T c2, c1, old[3];
-                if constexpr (PF) {
-                    c2 = cm[12], c1 = cm[13];
+                gptr<T> pp;
+                long e, s;
+                if (id < 2) {
+                    if (!nin[id]) continue;  // not a PML node of axis id
+                    e = (long)lpl[id] * (l - 1), s = lps[id];
+                    pp = lp[id];
+                    c2 = ln2[id], c1 = ln1[id];
+                    if constexpr (PF) {
 #pragma unroll
-                    for (int ic = 0; ic < 3; ic++) old[ic] = cm[9 + ic];
+                        for (int ic = 0; ic < 3; ic++) old[ic] = cm[7 + ic];
+                    } else {
+#pragma unroll
+                        for (int ic = 0; ic < 3; ic++) old[ic] = pp[e + ic * s];
+                    }
                 } else {
-                    c2 = P.dn2[pidx - 1], c1 = P.dn1[pidx - 1];
+                    const LSlot ps = pslot_l(l);
+                    if (!ps.on) continue;
+                    e = ps.e, s = ps.s;
+                    pp = ps.lo ? uniform_ptr(P.p[2]) : uniform_ptr(P.p[5]);
+                    if constexpr (PF) {
+                        c2 = cm[ZC + 2], c1 = cm[ZC + 3];
 #pragma unroll
-                    for (int ic = 0; ic < 3; ic++) old[ic] = pp[e + ic * stride];
+                        for (int ic = 0; ic < 3; ic++) old[ic] = cm[7 + ic];
+                    } else {
+                        c2 = P.dn2[ps.pidx - 1], c1 = P.dn1[ps.pidx - 1];
+#pragma unroll
+                        for (int ic = 0; ic < 3; ic++) old[ic] = pp[e + ic * s];
+                    }
                 }
 #pragma unroll
                 for (int ic = 0; ic < 3; ic++) {
                     const T sv = c2 * D[ic][id] + c1 * old[ic];
-                    pp[e + ic * stride] = old[ic] + P.dt * sv;
+                    pp[e + ic * s] = old[ic] + P.dt * sv;
                     D[ic][id] = sv;
                 }
             }
@@ -901,25 +1014,17 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
 #pragma unroll
             for (int ic = 0; ic < 3; ic++)  // reference order: id = ic, ic+1, ic+2
                 f[ic] = (D[ic][ic] + D[ic][(ic + 1) % 3]) + D[ic][(ic + 2) % 3];
-            bool interior = true;
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-                if ((MASK >> a & 1) && (n3[a] < P.i1pml[a] + 1 || n3[a] > P.i2pml[a] - 1)) interior = false;
+            bool interior = hint_jk;
+            if ((MASK & 4) && (l < P.i1pml[2] + 1 || l > P.i2pml[2] - 1)) interior = false;
 #pragma unroll
             for (int c = 0; c < 3; c++) f[c] = interior ? f[c] + Hf[c] : (MASK && P.hg_pml) ? f[c] + Hv[c] : f[c];
             if (fst.nodes) fs_force(P, j, k, l, f);
             // plane-local boundary conditions (sord/boundary_cond.f90:44-66)
+            const int z = zjk | bc_zero(2, l);
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                const int b = (a + 1) % 3, cc = (a + 2) % 3;
-                int bt = 0;
-                if (n3[a] == P.i1bc[a]) bt = P.bc1[a] % 10;
-                if (n3[a] == P.i2bc[a]) bt = P.bc2[a] % 10;
-                if (bt == -1) f[b] = f[cc] = T(0);
-                if (bt == 1) f[a] = T(0);
-                if (bt == 3) f[0] = f[1] = f[2] = T(0);
-            }
-            if (fa >= 0 && (n3[fa] == P.irup || n3[fa] == P.irup + 1)) {
+            for (int c = 0; c < 3; c++)
+                if (z >> c & 1) f[c] = T(0);
+            if (fjk || (fa == 2 && (l == P.irup || l == P.irup + 1))) {
                 // fault planes: the rupture kernel finishes these nodes
 #pragma unroll
                 for (int c = 0; c < 3; c++) P.a[c][o] = f[c];
@@ -947,8 +1052,8 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     // both planes of a pair always run (one loop back-edge keeps the register
     // sets in place); past the last plane of the item the second one stores nothing
     for (int l = wk.lz0; l <= wk.lz1; l += 2) {
-        plane(l, true, fA, mA, fB, mB, NB, NA, CA, CB);
-        plane(l + 1, l + 1 <= wk.lz1, fB, mB, fA, mA, NA, NB, CB, CA);
+        plane(l, true, fA, mA, fB, mB, SB, SA, CA, CB);
+        plane(l + 1, l + 1 <= wk.lz1, fB, mB, fA, mA, SA, SB, CB, CA);
     }
     if (stat_now) {
         st_a = wave_max(st_a);
@@ -962,8 +1067,10 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     }
 }
 
+// FP32: at most 256 VGPRs, two waves per SIMD
 template <class T, int W, int MASK>
-__global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+void k_fusedp(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;  // uniform over the block
     __shared__ T lds[boundary_lds_slots<W>()];
@@ -971,14 +1078,16 @@ __global__ __launch_bounds__(64 * W) void k_fusedp(Params<T> P, FusedC<T> C, Poi
 }
 // a finite source: every boundary item on the kernel of all damped axes
 template <class T, int W>
-__global__ __launch_bounds__(64 * W) void k_fusedp_fs(Params<T> P, FusedC<T> C, PointOps<T> ops) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+void k_fusedp_fs(Params<T> P, FusedC<T> C, PointOps<T> ops) {
     const Work wk = C.work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;
     __shared__ T lds[boundary_lds_slots<W>()];
     boundary_item<T, W, 7, true>(P, C, ops, wk, lds);
 }
 template <class T, int W, int MASK>
-__global__ __launch_bounds__(64 * W) void k_fusedp_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+void k_fusedp_batch(const Scen<T> *__restrict__ sc, const Work *__restrict__ work) {
     const Work wk = work[blockIdx.x];
     if (wk.lz0 > wk.lz1) return;
     const Scen<T> &S = sc[wk.pad1];
