@@ -218,6 +218,9 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     fetch_node(wk.lz0 + 1, RA);
     fetch_cell(wk.lz0, mA);
     cell_at(wk.lz0 - 1, mB, NA, NB, EA, CeA, HA, ChA, false);
+    // every wave has read the second plane from LDS before the loop's first
+    // plane writes its own slots again
+    __syncthreads();
 
     auto plane = [&](int l, bool act, const T (&Rc)[ADJ_NR], const T (&mc)[6], T (&Rn)[ADJ_NR], T (&mn)[6],
                      const T (&Nlo)[4][9], T (&Nhi)[4][9], const T (&cE)[3][3], const T (&cH)[3][3], T (&nE)[3][3],

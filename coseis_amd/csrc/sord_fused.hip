@@ -367,6 +367,10 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
         fetch_cell(wk.lz0, mA);
     }
     cell_at(wk.lz0 - 1, mB, NA, NB, E, CA, true);
+    // every wave has read the second plane from LDS before the first plane of
+    // the loop writes its own slots again (a wave that ran ahead would
+    // overwrite the row its neighbour below has not read yet)
+    __syncthreads();
 
     // one plane: cu..cm hold plane l+1 and cell l (arrived), nu..nm receive plane
     // l+2 and cell l+1; Nlo holds plane l, Nhi receives plane l+1; carry holds
@@ -920,6 +924,9 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
     fetch_node(wk.lz0 + 1, fA);
     fetch_cell(wk.lz0, mA);
     cell_at(wk.lz0 - 1, mB, SA, SB, E, CA, false, true);
+    // every wave has read the second plane from LDS before the loop's first
+    // plane writes its own slots again
+    __syncthreads();
 
     auto plane = [&](int l, bool act, const T (&cf)[7], const T (&cm)[NMAT], T (&nf)[7], T (&nm_)[NMAT],
                      const PlaneSums<T> &Slo, PlaneSums<T> &Shi, const T (&carry)[3][9], T (&Cn)[3][9]) {
