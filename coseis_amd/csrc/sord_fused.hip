@@ -191,20 +191,23 @@ __device__ __forceinline__ void fs_force(const Params<T> &P, int j, int k, int l
     for (int c = 0; c < 3; c++) f[c] = f[c] + P.fs[c][q];
 }
 
-// the four hourglass modes of one component over the cell corners
-// (F*lo: corners (dj,dk) of plane l, F*hi of plane l+1, index dj + 2 dk)
+// Sums over the four nodes (j, k), (j+1, k), (j, k+1), (j+1, k+1) of one node
+// plane, from which a cell's differences and hourglass modes follow: each
+// plane's sums serve the cell below it and the cell above it.  With f_q the
+// node values (q = dj + 2 dk):
+//   J = (f1 - f0) + (f3 - f2)   K = (f2 - f0) + (f3 - f1)
+//   S = f0 + f1 + f2 + f3       X = f0 - f1 - f2 + f3
+// A cell between planes lo and hi has the differences along j, k, l
+// J_lo + J_hi, K_lo + K_hi, S_hi - S_lo, and the hourglass modes
+// (s_k s_l, s_j s_l, s_j s_k, s_j s_k s_l) K_hi - K_lo, J_hi - J_lo,
+// X_lo + X_hi, X_lo - X_hi (s = +1 at the lower corner).
 template <class T>
-__device__ __forceinline__ void hg_modes(T f00, T f10, T f01, T f11, T g00, T g10, T g01, T g11, T h[4]) {
-    const T a00 = f00 - f10, a10 = f01 - f11;  // along j, at (dk, dl) = (0,0), (1,0)
-    const T a01 = g00 - g10, a11 = g01 - g11;  // (0,1), (1,1)
-    const T b00 = f00 + f10, b10 = f01 + f11;
-    const T b01 = g00 + g10, b11 = g01 + g11;
-    h[0] = (b00 - b10) - (b01 - b11);  // s_k s_l
-    h[1] = (a00 + a10) - (a01 + a11);  // s_j s_l
-    const T d0 = a00 - a10, d1 = a01 - a11;
-    h[2] = d0 + d1;  // s_j s_k
-    h[3] = d0 - d1;  // s_j s_k s_l
-}
+struct PlaneSums {
+    T w[3][3];  // u + gamma v: J, K, S (strain)
+    T F[3][3];  // h1 u + dt h2 v: J, K, X (full hourglass)
+    T v[3][4];  // v: J, K, S (strain in a damped direction), X (viscous hourglass)
+    T u[3][3];  // u: J, K, S (strain in the other directions of a PML slab)
+};
 
 // ---------------------------------------------------------------------------
 // interior tiles
@@ -260,36 +263,36 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
         const long o = g.at(jc, kc, min(l, lmax));
         mat[0] = P.lam[o], mat[1] = P.mu[o], mat[2] = P.yy[o], mat[3] = P.mr[o];
     };
-    auto exchange = [&](const T own6[6], T N[4][6]) {
+    // the plane sums of W (J, K, S) and F (J, K, X): the row above through
+    // LDS, the next column through a lane shift of the row pair's sum and
+    // difference
+    auto exchange = [&](const T own6[6], PlaneSums<T> &S) {
 #pragma unroll
         for (int q = 0; q < 6; q++) sN[q][w][lane] = own6[q];
         __syncthreads();
         const int wa = w + 1 < W ? w + 1 : w;
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            const T ab = sN[q][wa][lane];
-            N[0][q] = own6[q];
-            N[2][q] = ab;
-            N[1][q] = lane_next(own6[q]);
-            N[3][q] = lane_next(ab);
+            // the own value's shift does not wait for the LDS read
+            const T ab = sN[q][wa][lane], on = lane_next(own6[q]), an = lane_next(ab);
+            const T e = own6[q] + ab, d = ab - own6[q], ne = on + an, nd = an - on;
+            if (q < 3) S.w[q][0] = ne - e, S.w[q][1] = d + nd, S.w[q][2] = e + ne;
+            else S.F[q - 3][0] = ne - e, S.F[q - 3][1] = d + nd, S.F[q - 3][2] = nd - d;
         }
     };
-    // stress and hourglass of cell (j, k, l) from the corner data of planes l
-    // (Nlo) and l+1 (Nhi); E: polynomial at s_l = +1, Cc: at s_l = -1
+    // stress and hourglass of cell (j, k, l) from the plane sums of planes l
+    // (Slo) and l+1 (Shi); E: polynomial at s_l = +1, Cc: at s_l = -1
     // st: the cell counts in the statistics (not the spare plane past the item's
     // last, which may lie where another kernel's damping applies)
-    auto cell_at = [&](int l, const T mat[4], const T (&Nlo)[4][6], const T (&Nhi)[4][6], T E[3][4], T Cc[3][4],
-                       bool st) {
+    auto cell_at = [&](int l, const T mat[4], const PlaneSums<T> &Slo, const PlaneSums<T> &Shi, T E[3][4],
+                       T Cc[3][4], bool st) {
         const T lm = mat[0], m = mat[1], y = mat[2];
         T G[3][3];  // G[a][c]: gradient along a of component c
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            G[0][c] = C.gx * (((Nlo[1][c] - Nlo[0][c]) + (Nlo[3][c] - Nlo[2][c])) +
-                              ((Nhi[1][c] - Nhi[0][c]) + (Nhi[3][c] - Nhi[2][c])));
-            G[1][c] = C.gy * (((Nlo[2][c] - Nlo[0][c]) + (Nlo[3][c] - Nlo[1][c])) +
-                              ((Nhi[2][c] - Nhi[0][c]) + (Nhi[3][c] - Nhi[1][c])));
-            G[2][c] = C.gz * (((Nhi[0][c] - Nlo[0][c]) + (Nhi[1][c] - Nlo[1][c])) +
-                              ((Nhi[2][c] - Nlo[2][c]) + (Nhi[3][c] - Nlo[3][c])));
+            G[0][c] = C.gx * (Slo.w[c][0] + Shi.w[c][0]);
+            G[1][c] = C.gy * (Slo.w[c][1] + Shi.w[c][1]);
+            G[2][c] = C.gz * (Shi.w[c][2] - Slo.w[c][2]);
         }
         T s[6];
         // source ops: the boxes met by this tile, then those of plane l (both
@@ -323,9 +326,12 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
                            {C.ax * s[4], C.ay * s[3], C.az * s[2]}};
 #pragma unroll
         for (int c = 0; c < 3; c++) {
+            // modes (s_k s_l, s_j s_l, s_j s_k, s_j s_k s_l) of F
             T h[4];
-            hg_modes<T>(Nlo[0][3 + c], Nlo[1][3 + c], Nlo[2][3 + c], Nlo[3][3 + c], Nhi[0][3 + c], Nhi[1][3 + c],
-                        Nhi[2][3 + c], Nhi[3][3 + c], h);
+            h[0] = Shi.F[c][1] - Slo.F[c][1];
+            h[1] = Shi.F[c][0] - Slo.F[c][0];
+            h[2] = Slo.F[c][2] + Shi.F[c][2];
+            h[3] = Slo.F[c][2] - Shi.F[c][2];
 #pragma unroll
             for (int q = 0; q < 4; q++) h[q] = y * h[q];
             E[c][0] = A[c][2];
@@ -344,7 +350,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     // register holding a load in flight is ever copied (a copy would make the
     // compiler wait for the load at the end of every iteration).
     T own6[6], E[3][4];
-    T NA[4][6], NB[4][6];  // corner data of two planes (lower, upper in turn)
+    PlaneSums<T> NA, NB;  // plane sums of two planes (lower, upper in turn)
     T CA[3][4], CB[3][4];  // s_l = -1 polynomials of two cells (carried, new in turn)
     T uA[3], vA[3], gA, mA[4], uB[3], vB[3], gB, mB[4];  // loads: plane l+1 raw, cell l + 1/m
     T uC[3], vC[3], gC, mC[4];                            // the third set (DEPTH 2)
@@ -376,7 +382,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     // l+2 and cell l+1; Nlo holds plane l, Nhi receives plane l+1; carry holds
     // cell l-1's polynomial at s_l = -1, Cn receives cell l's
     auto plane = [&](int l, bool act, T (&cu)[3], T (&cv)[3], T &cg, T (&cm)[4], T (&nu)[3], T (&nv)[3], T &ng,
-                     T (&nm)[4], T (&Nlo)[4][6], T (&Nhi)[4][6], T (&carry)[3][4], T (&Cn)[3][4]) {
+                     T (&nm)[4], const PlaneSums<T> &Nlo, PlaneSums<T> &Nhi, T (&carry)[3][4], T (&Cn)[3][4]) {
         node_vals(cu, cv, cg, own6);
         fetch_node(l + 1 + DEPTH, nu, nv, ng);
         fetch_cell(l + DEPTH, nm);
@@ -529,23 +535,6 @@ void k_fused_batch_stat(const Scen<T> *__restrict__ sc, const Work *__restrict__
 template <int W>
 constexpr int boundary_lds_slots() { return 33 * (W * 64 + 64); }
 
-// Sums over the four nodes (j, k), (j+1, k), (j, k+1), (j+1, k+1) of one node
-// plane, from which a cell's differences and hourglass modes follow: each
-// plane's sums serve the cell below it and the cell above it.  With f_q the
-// node values (q = dj + 2 dk):
-//   J = (f1 - f0) + (f3 - f2)   K = (f2 - f0) + (f3 - f1)
-//   S = f0 + f1 + f2 + f3       X = f0 - f1 - f2 + f3
-// A cell between planes lo and hi has the differences along j, k, l
-// J_lo + J_hi, K_lo + K_hi, S_hi - S_lo, and the hourglass modes
-// (s_k s_l, s_j s_l, s_j s_k, s_j s_k s_l) K_hi - K_lo, J_hi - J_lo,
-// X_lo + X_hi, X_lo - X_hi (s = +1 at the lower corner).
-template <class T>
-struct PlaneSums {
-    T w[3][3];  // u + gamma v: J, K, S (strain)
-    T F[3][3];  // h1 u + dt h2 v: J, K, X (full hourglass)
-    T v[3][4];  // v: J, K, S (strain in a damped direction), X (viscous hourglass)
-    T u[3][3];  // u: J, K, S (strain in the other directions of a PML slab)
-};
 
 template <class T, int W, int MASK, bool FS>
 __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T> &C, const PointOps<T> &ops,
