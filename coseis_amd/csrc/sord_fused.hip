@@ -220,9 +220,6 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
                                               const Work &wk) {
     __shared__ T sN[6][W][64];
     __shared__ T sC[6][W][64];
-    // statistics maxima of each lane (|a|^2, |v|^2, |u|^2, |w|^2), kept in LDS
-    // rather than in registers, which the plane loop has none to spare for
-    __shared__ T sS[ST ? 4 : 1][W][64];
     const int lane = threadIdx.x, w = threadIdx.y;
     const int j = wk.jb - 1 + lane, k = wk.kb - 1 + w;
     const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= W - 2 && j <= wk.je && k <= wk.ke;
@@ -234,11 +231,10 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
     const bool stat_now = ST && P.stat_now, stat_next = ST && P.stat_next;
     const unsigned tile_boxes = ops.n ? ops_boxes_jk(ops, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : 0u;
     const FsTile fst = FS ? fs_tile(P, wk.jb - 1, wk.je, wk.kb - 1, wk.ke) : FsTile{false, false};
-    if (ST)
-#pragma unroll
-        for (int q = 0; q < 4; q++) sS[ST ? q : 0][w][lane] = T(0);
+    // statistics maxima of each lane (|a|^2, |v|^2, |u|^2, |w|^2)
+    T stm[4] = {T(0), T(0), T(0), T(0)};
     auto st_max = [&](int q, T x) {
-        if (ST) sS[ST ? q : 0][w][lane] = tmax(sS[ST ? q : 0][w][lane], x);
+        if (ST) stm[q] = tmax(stm[q], x);
     };
 
     // node data: W = u + gamma v (strain), F = h1 u + dt h2 v (hourglass)
@@ -449,13 +445,13 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
         }
     }
     if (stat_now) {
-        const T st_a = wave_max(sS[0][w][lane]);
-        const T st_w = wave_max(sS[ST ? 3 : 0][w][lane]);
+        const T st_a = wave_max(stm[0]);
+        const T st_w = wave_max(stm[3]);
         if (lane == 0) atomic_max_bits<T>(P.stat_cur + 0, st_a), atomic_max_bits<T>(P.stat_cur + 3, st_w);
     }
     if (stat_next) {
-        const T st_v = wave_max(sS[ST ? 1 : 0][w][lane]);
-        const T st_u = wave_max(sS[ST ? 2 : 0][w][lane]);
+        const T st_v = wave_max(stm[1]);
+        const T st_u = wave_max(stm[2]);
         if (lane == 0) atomic_max_bits<T>(P.stat_nxt + 1, st_v), atomic_max_bits<T>(P.stat_nxt + 2, st_u);
     }
 }

@@ -278,6 +278,7 @@ struct Solver {
     ~Solver() {
         if (stream) (void)hipStreamSynchronize(stream);  // the side streams join the main stream every step
         for (void *p : dallocs) (void)hipFree(p);
+        if (hstat) (void)hipHostFree(hstat);
         for (int i = 0; i < NSIDE; i++) {
             if (side[i] && own_streams) (void)hipStreamDestroy(side[i]);
             if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
@@ -1074,12 +1075,15 @@ extern "C" int sord_bench_boundary(void *h, int nsteps, int *nlaunch, int *masks
             throw;
         }
         S.bench_bnd_on = false;
-        *nlaunch = (int)S.blaunch.size();
-        for (int q = 0; q < *nlaunch && q < cap; q++) {
-            masks[q] = S.blaunch[q].mask;
-            ms[q] = q < (int)S.bench_bnd_ms.size() ? S.bench_bnd_ms[q] : 0.0;
-            nodes[q] = S.blaunch[q].nodes;
-            damped[q] = S.blaunch[q].damped;
+        // entry 0: the interior launch (mask -1), then the boundary launches
+        *nlaunch = 1 + (int)S.blaunch.size();
+        if (cap > 0) masks[0] = -1, ms[0] = kern, nodes[0] = (double)S.bench_nodes(), damped[0] = 0.0;
+        for (int q = 1; q < *nlaunch && q < cap; q++) {
+            const int b = q - 1;
+            masks[q] = S.blaunch[b].mask;
+            ms[q] = b < (int)S.bench_bnd_ms.size() ? S.bench_bnd_ms[b] : 0.0;
+            nodes[q] = S.blaunch[b].nodes;
+            damped[q] = S.blaunch[b].damped;
         }
     });
 }

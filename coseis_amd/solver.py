@@ -175,8 +175,9 @@ class Solver(object):
         return it.value, nt.value
 
     def bench(self, nsteps):
-        """Advance nsteps timed: (ms all steps, ms per dominant-kernel launch,
-        nodes per launch, kernel name)."""
+        """Advance nsteps timed: (ms all steps, 0 (the launches are not
+        instrumented; see bench_boundary), nodes per dominant-kernel launch,
+        kernel name)."""
         tot, ker = ctypes.c_double(), ctypes.c_double()
         nodes = ctypes.c_int64()
         name = ctypes.create_string_buffer(64)
@@ -185,8 +186,9 @@ class Solver(object):
         return tot.value, ker.value, nodes.value, name.value.decode()
 
     def bench_boundary(self, nsteps):
-        """Advance nsteps steps timing each boundary launch (one stream):
-        [(mask, mean ms, nodes, damped-axis memberships)] per launch of a step
+        """Advance nsteps steps timing the interior launch and each boundary
+        launch (one stream): [(mask, mean ms, nodes, damped-axis memberships)]
+        per launch of a step, the interior launch first with mask -1
         (sord_bench_boundary)."""
         n = ctypes.c_int()
         cap = 64

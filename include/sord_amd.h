@@ -99,22 +99,25 @@ int sord_set_tensor(void *handle, const char *name, void *dev_ptr, int64_t numel
 /* Elements of one padded solver array (the size sord_set_tensor needs). */
 int sord_tensor_numel(void *handle, int64_t *numel);
 
-/* Advance `nsteps` steps (with their field i/o) timed with HIP events on the
- * library's stream.  Returns the elapsed time of all steps (ms), the mean
- * duration of one launch of the dominant solver kernel (ms; a pair of events
- * that each launch stamps with the kernel's start and end), the number of
- * nodes that launch advances, and the kernel's name.  No counterpart in the
- * reference: it is the measurement hook of the benchmark. */
+/* Advance `nsteps` steps (with their field i/o) timed with a pair of HIP
+ * events on the library's stream around all of them.  Returns the elapsed
+ * time of all steps (ms), 0 for ms_kernel (the launches are not instrumented
+ * here: a launch that records its own start and end adds gaps to the step;
+ * per-launch durations come from sord_bench_boundary), the number of nodes the
+ * dominant solver kernel's launch advances, and the kernel's name.  No
+ * counterpart in the reference: it is the measurement hook of the benchmark. */
 int sord_bench(void *handle, int nsteps, double *ms_total, double *ms_kernel, int64_t *kernel_nodes,
                char *kernel_name, int name_len);
 
-/* Advance nsteps steps as sord_bench does, with every boundary launch
- * stamping a pair of HIP events on the solver stream (one stream only: SORD_ESTATE
- * with side streams).  Returns the number of boundary launches per step and,
- * in [0, cap), each one's damped-axis specialisation, mean duration (ms), the
- * nodes it advances and the sum over them of the axes along which the node is
- * damped (the PML memory it reads and writes).  No counterpart in the
- * reference: the measurement hook of the boundary kernels' roofline. */
+/* Advance nsteps steps as sord_bench does, with the interior launch and
+ * every boundary launch stamping a pair of HIP events with the kernel's start
+ * and end on the solver stream (one stream only: SORD_ESTATE with side
+ * streams).  Returns the number of launches per step and, in [0, cap), each
+ * one's damped-axis specialisation (entry 0: the interior launch, mask -1),
+ * mean duration (ms), the nodes it advances and the sum over them of the axes
+ * along which the node is damped (the PML memory it reads and writes).  No
+ * counterpart in the reference: the measurement hook of the per-launch
+ * rooflines. */
 int sord_bench_boundary(void *handle, int nsteps, int *nlaunch, int *masks, double *ms, double *nodes,
                         double *damped, int cap);
 
