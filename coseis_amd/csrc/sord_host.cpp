@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <string>
 #include <vector>
@@ -157,6 +158,13 @@ struct IoRec {
     T *dbuf = nullptr;     // device staging (nb records) for device outputs
     size_t dcap = 0;       // its capacity (values)
     size_t rec = 0;
+    // a device output's flush: its records copied into pinned host memory,
+    // written to the file once the copy has completed (io_poll)
+    T *pin = nullptr;
+    size_t pcap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    int p_o = 0, p_n = 0;  // record offset and count of the pending write
 };
 
 // list-directed tokens of one line
@@ -277,6 +285,14 @@ struct Solver {
 
     ~Solver() {
         if (stream) (void)hipStreamSynchronize(stream);  // the side streams join the main stream every step
+        try {
+            io_complete_all();
+        } catch (...) {
+        }
+        for (auto &o : io) {
+            if (o.pin) (void)hipHostFree(o.pin);
+            if (o.ev) (void)hipEventDestroy(o.ev);
+        }
         for (void *p : dallocs) (void)hipFree(p);
         if (hstat) (void)hipHostFree(hstat);
         for (int i = 0; i < NSIDE; i++) {
@@ -665,12 +681,66 @@ struct Solver {
         return true;
     }
 
-    // buffered output flush (sord/input_output.f90:494-508)
+    // device outputs whose copies are in flight, in the order of their
+    // flushes (the file writes keep that order)
+    std::deque<size_t> io_fifo;
+    void io_write_pending(IoRec<T> &o) {
+        if (o.rec >= 1 && o.p_n >= 1) file_write(o.fname, o.pin, o.rec, o.p_o, o.p_n);
+        o.pending = false;
+    }
+    // write the pending flushes whose copies have completed (without waiting)
+    void io_poll() {
+        while (!io_fifo.empty()) {
+            IoRec<T> &o = io[io_fifo.front()];
+            const hipError_t e = hipEventQuery(o.ev);
+            if (e == hipErrorNotReady) return;
+            HIPCHECK(e);
+            io_fifo.pop_front();
+            io_write_pending(o);
+        }
+    }
+    // write every pending flush (before returning to the caller, who may read
+    // the files, and before a pinned buffer is reused)
+    void io_complete_all() {
+        while (!io_fifo.empty()) {
+            IoRec<T> &o = io[io_fifo.front()];
+            HIPCHECK(hipEventSynchronize(o.ev));
+            io_fifo.pop_front();
+            io_write_pending(o);
+        }
+    }
+
+    // buffered output flush (sord/input_output.f90:494-508).  A device output
+    // is copied into pinned host memory on the stream and written once the
+    // copy has completed, so a flush does not drain the stream; in the
+    // allocation-free step (no_alloc) it is copied and written at once.
     void io_flush(IoRec<T> &o, Index &x, bool force_close) {
         const int it1 = o.ii[3][0], it2 = o.ii[3][1], dit = o.ii[3][2];
         if (o.ib == o.nb || it == it2 || fmodulo(it, itio) == 0 || force_close) {
             x.n[3] = o.ib;
             x.o[3] = (it - it1) / dit + 1 - x.n[3];
+            if (o.dbuf && !no_alloc) {
+                if (o.pending) io_complete_all();  // its pinned buffer is reused
+                if (o.pcap < o.rec * o.nb) {
+                    if (o.pin) HIPCHECK(hipHostFree(o.pin));
+                    o.pin = nullptr;
+                    HIPCHECK(hipHostMalloc((void **)&o.pin, std::max<size_t>(1, o.rec * o.nb) * sizeof(T),
+                                           hipHostMallocDefault));
+                    o.pcap = o.rec * o.nb;
+                }
+                if (!o.ev) HIPCHECK(hipEventCreateWithFlags(&o.ev, hipEventDisableTiming));
+                HIPCHECK(hipMemcpyAsync(o.pin, o.dbuf, o.rec * o.ib * sizeof(T), hipMemcpyDeviceToHost, stream));
+                HIPCHECK(hipEventRecord(o.ev, stream));
+                o.pending = true, o.p_o = x.o[3], o.p_n = x.n[3];
+                io_fifo.push_back((size_t)(&o - io.data()));
+                o.ib = 0;
+                if (it == it2 || o.rec < 1) {
+                    o.hbuf.clear();
+                    o.op[0] = '#';
+                }
+                return;
+            }
+            io_complete_all();  // the writes keep the order of the flushes
             if (o.dbuf) {
                 o.hbuf.resize(o.rec * o.nb);
                 HIPCHECK(hipMemcpyAsync(o.hbuf.data(), o.dbuf, o.rec * o.ib * sizeof(T), hipMemcpyDeviceToHost,
@@ -860,8 +930,15 @@ static int dispatch(void *h, Fn &&fn) {
         return SORD_ESTATE;
     }
     Handle *H = (Handle *)h;
-    if (H->realsize == 4) return guard([&] { fn(*H->s4); });
-    return guard([&] { fn(*H->s8); });
+    // the files of the outputs flushed during the call are complete on return
+    if (H->realsize == 4) return guard([&] {
+        fn(*H->s4);
+        if (H->s4->initialized) H->s4->io_complete_all();
+    });
+    return guard([&] {
+        fn(*H->s8);
+        if (H->s8->initialized) H->s8->io_complete_all();
+    });
 }
 
 #include "sord_batch.inc"
@@ -879,8 +956,17 @@ static int dispatch_batch(void *h, Fn &&fn) {
         return SORD_ESTATE;
     }
     BatchHandle *H = (BatchHandle *)h;
-    if (H->realsize == 4) return guard([&] { fn(*H->b4); });
-    return guard([&] { fn(*H->b8); });
+    // the files of the outputs flushed during the call are complete on return
+    if (H->realsize == 4) return guard([&] {
+        fn(*H->b4);
+        for (auto &s : H->b4->S)
+            if (s->initialized) s->io_complete_all();
+    });
+    return guard([&] {
+        fn(*H->b8);
+        for (auto &s : H->b8->S)
+            if (s->initialized) s->io_complete_all();
+    });
 }
 
 }  // namespace sord
