@@ -18,9 +18,10 @@
 // One fused kernel per adjoint step, on the z-march tiles of the forward
 // interior kernel (sord_fused.hip): a block of W waves marches the planes l
 // of its tile; per plane every lane computes the cell whose lower corner is
-// its node from the 8 corner values of r, w and F (exchanged through LDS and
-// DPP lane shifts), and the cell's contributions to its corners are gathered
-// as polynomials in the corner signs, carried from plane to plane.  Ke r and
+// its node from the plane sums of r, w and F (the row above through LDS, the
+// next column through DPP lane shifts), and the cell's contributions to its
+// corners are gathered as polynomials in the corner signs, carried from plane
+// to plane.  Ke r and
 // Kh r are gathered separately (they enter ub and vb with different weights).
 // Cells outside the mesh contribute nothing (free surfaces); the material
 // sums of a cell are accumulated by the block that owns its lower corner.
@@ -45,21 +46,20 @@ __device__ __forceinline__ double adj_prev(double x) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// the four hourglass modes over the cell corners (planes l: f, l+1: g;
-// corner index dj + 2 dk; s = +1 at the lower corner): s_k s_l, s_j s_l,
-// s_j s_k, s_j s_k s_l
+// Sums over the four nodes (j, k), (j+1, k), (j, k+1), (j+1, k+1) of one node
+// plane (f_q, q = dj + 2 dk), shared by the cell below and the cell above the
+// plane:  J = (f1 - f0) + (f3 - f2),  K = (f2 - f0) + (f3 - f1),
+// S = f0 + f1 + f2 + f3,  X = f0 - f1 - f2 + f3.  A cell between planes lo
+// and hi has the differences along j, k, l J_lo + J_hi, K_lo + K_hi,
+// S_hi - S_lo and the hourglass modes (s_k s_l, s_j s_l, s_j s_k,
+// s_j s_k s_l; s = +1 at the lower corner) K_hi - K_lo, J_hi - J_lo,
+// X_lo + X_hi, X_lo - X_hi.
 template <class T>
-__device__ __forceinline__ void adj_modes(const T f[4], const T g[4], T h[4]) {
-    const T a00 = f[0] - f[1], a10 = f[2] - f[3];
-    const T a01 = g[0] - g[1], a11 = g[2] - g[3];
-    const T b00 = f[0] + f[1], b10 = f[2] + f[3];
-    const T b01 = g[0] + g[1], b11 = g[2] + g[3];
-    h[0] = (b00 - b10) - (b01 - b11);
-    h[1] = (a00 + a10) - (a01 + a11);
-    const T d0 = a00 - a10, d1 = a01 - a11;
-    h[2] = d0 + d1;
-    h[3] = d0 - d1;
-}
+struct AdjSums {
+    T r[3][4];  // r: J, K, S, X
+    T w[3][3];  // w: J, K, S
+    T F[3][3];  // F: J, K, X
+};
 
 // raw node loads of one plane: ub 0-2, vb 3-5, u 6-8, v 9-11, v of the next
 // state 12-14, 1/m 15, gamma 16, the node's gradient sum 17.  The gradient
@@ -111,40 +111,38 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
             o9[6 + i] = h1 * R[6 + i] + dth2 * R[9 + i];
         }
     };
-    auto exchange = [&](const T (&o9)[9], T (&N)[4][9]) {
+    // the plane sums of r, w, F: the row above through LDS, the next column
+    // through lane shifts (the own value's before the LDS read)
+    auto exchange = [&](const T (&o9)[9], AdjSums<T> &N) {
 #pragma unroll
         for (int q = 0; q < 9; q++) sN[q][w][lane] = o9[q];
         __syncthreads();
         const int wa = w + 1 < W ? w + 1 : w;
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            const T ab = sN[q][wa][lane];
-            N[0][q] = o9[q];
-            N[2][q] = ab;
-            N[1][q] = adj_next(o9[q]);
-            N[3][q] = adj_next(ab);
+            const T ab = sN[q][wa][lane], on = adj_next(o9[q]), an = adj_next(ab);
+            const T e = o9[q] + ab, d = ab - o9[q], ne = on + an, nd = an - on;
+            const T J = ne - e, K = d + nd, S = e + ne, X = nd - d;
+            const int f = q / 3, c = q % 3;
+            if (f == 0) N.r[c][0] = J, N.r[c][1] = K, N.r[c][2] = S, N.r[c][3] = X;
+            else if (f == 1) N.w[c][0] = J, N.w[c][1] = K, N.w[c][2] = S;
+            else N.F[c][0] = J, N.F[c][1] = K, N.F[c][2] = X;
         }
     };
     // cell (j, k, l): polynomials of Ke r (E: 1, s_j, s_k at s_l = +1; Ce at
     // s_l = -1) and of Kh r (H: s_j, s_k, s_j s_k; Ch), material sums if acc
-    auto cell_at = [&](int l, const T (&M)[6], const T (&Nlo)[4][9], const T (&Nhi)[4][9], T (&E)[3][3],
+    auto cell_at = [&](int l, const T (&M)[6], const AdjSums<T> &Nlo, const AdjSums<T> &Nhi, T (&E)[3][3],
                        T (&Ce)[3][3], T (&H)[3][3], T (&Ch)[3][3], bool acc) {
         const bool real = colreal && l >= A.lo[2] && l <= A.hi[2];
         T Gr[3][3], G[3][3];  // [a][c]: derivative along a of component c
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-#pragma unroll
-            for (int f = 0; f < 2; f++) {
-                const int q = f ? 3 + c : c;
-                const T gxv = A.rd[0] * (((Nlo[1][q] - Nlo[0][q]) + (Nlo[3][q] - Nlo[2][q])) +
-                                         ((Nhi[1][q] - Nhi[0][q]) + (Nhi[3][q] - Nhi[2][q])));
-                const T gyv = A.rd[1] * (((Nlo[2][q] - Nlo[0][q]) + (Nlo[3][q] - Nlo[1][q])) +
-                                         ((Nhi[2][q] - Nhi[0][q]) + (Nhi[3][q] - Nhi[1][q])));
-                const T gzv = A.rd[2] * (((Nhi[0][q] - Nlo[0][q]) + (Nhi[1][q] - Nlo[1][q])) +
-                                         ((Nhi[2][q] - Nlo[2][q]) + (Nhi[3][q] - Nlo[3][q])));
-                if (f) G[0][c] = gxv, G[1][c] = gyv, G[2][c] = gzv;
-                else Gr[0][c] = gxv, Gr[1][c] = gyv, Gr[2][c] = gzv;
-            }
+            Gr[0][c] = A.rd[0] * (Nlo.r[c][0] + Nhi.r[c][0]);
+            Gr[1][c] = A.rd[1] * (Nlo.r[c][1] + Nhi.r[c][1]);
+            Gr[2][c] = A.rd[2] * (Nhi.r[c][2] - Nlo.r[c][2]);
+            G[0][c] = A.rd[0] * (Nlo.w[c][0] + Nhi.w[c][0]);
+            G[1][c] = A.rd[1] * (Nlo.w[c][1] + Nhi.w[c][1]);
+            G[2][c] = A.rd[2] * (Nhi.w[c][2] - Nlo.w[c][2]);
         }
         const T lm = real ? M[0] : T(0), m = real ? M[1] : T(0), y = real ? M[2] : T(0);
         const T trr = Gr[0][0] + Gr[1][1] + Gr[2][2];
@@ -162,13 +160,11 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
         T sy = T(0);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            T fl[4], fh[4], hr[4], hf[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) fl[q] = Nlo[q][c], fh[q] = Nhi[q][c];
-            adj_modes<T>(fl, fh, hr);
-#pragma unroll
-            for (int q = 0; q < 4; q++) fl[q] = Nlo[q][6 + c], fh[q] = Nhi[q][6 + c];
-            adj_modes<T>(fl, fh, hf);
+            // modes (s_k s_l, s_j s_l, s_j s_k, s_j s_k s_l) of r and of F
+            const T hr[4] = {Nhi.r[c][1] - Nlo.r[c][1], Nhi.r[c][0] - Nlo.r[c][0], Nlo.r[c][3] + Nhi.r[c][3],
+                             Nlo.r[c][3] - Nhi.r[c][3]};
+            const T hf[4] = {Nhi.F[c][1] - Nlo.F[c][1], Nhi.F[c][0] - Nlo.F[c][0], Nlo.F[c][2] + Nhi.F[c][2],
+                             Nlo.F[c][2] - Nhi.F[c][2]};
 #pragma unroll
             for (int q = 0; q < 4; q++) sy = sy + hr[q] * hf[q];
             T h[4];
@@ -194,7 +190,7 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     };
 
     T o9[9];
-    T NA[4][9], NB[4][9];
+    AdjSums<T> NA, NB;  // plane sums of two planes (lower, upper in turn)
     T EA[3][3], HA[3][3];                              // polynomials of the current cell (s_l = +1)
     T CeA[3][3], ChA[3][3], CeB[3][3], ChB[3][3];      // s_l = -1 polynomials (carried, new in turn)
     T RA[ADJ_NR], RB[ADJ_NR], mA[6], mB[6];
@@ -223,7 +219,7 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     __syncthreads();
 
     auto plane = [&](int l, bool act, const T (&Rc)[ADJ_NR], const T (&mc)[6], T (&Rn)[ADJ_NR], T (&mn)[6],
-                     const T (&Nlo)[4][9], T (&Nhi)[4][9], const T (&cE)[3][3], const T (&cH)[3][3], T (&nE)[3][3],
+                     const AdjSums<T> &Nlo, AdjSums<T> &Nhi, const T (&cE)[3][3], const T (&cH)[3][3], T (&nE)[3][3],
                      T (&nH)[3][3]) {
         node_vals(Rc, o9);
         fetch_node(l + 2, Rn);
@@ -256,12 +252,20 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
         }
         if (own && act) {
             const long o = g.at(j, k, l);
+            // the residual at receiver nodes of this tile (uniform loop)
+            T inj[3] = {T(0), T(0), T(0)};
+            for (int e = wk.pad0; e < wk.pad0 + wk.pad1; e++) {
+                const int4 r = A.rtile[e];
+                if (r.x == j && r.y == k && r.z == l)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) inj[c] = inj[c] + A.res[3 * r.w + c];
+            }
             T gm = T(0);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const T vbp = ow[3 + c] + dt * ow[c];
                 A.uo[c][o] = ow[c] + fe[c] + h1 * fh[c];
-                A.vo[c][o] = vbp + ow[13] * fe[c] + dth2 * fh[c];
+                A.vo[c][o] = (vbp + ow[13] * fe[c] + dth2 * fh[c]) + inj[c];
                 gm = gm + vbp * (ow[9 + c] - ow[6 + c]);
             }
             if (ow[12] != T(0)) A.gm[o] = ow[14] + gm / ow[12];
@@ -294,6 +298,20 @@ __global__ void k_rec_gather(const T *v0, const T *v1, const T *v2, Grid<T> g, c
     out[3 * q] = v0[o], out[3 * q + 1] = v1[o], out[3 * q + 2] = v2[o];
 }
 
+// velocity at receiver nodes of many states (one thread per state and
+// receiver): out[s][q][c] = v_c of states[s] at receiver q
+template <class T>
+__global__ void k_rec_gather_states(const T *const *states, int nstate, size_t S, Grid<T> g, const int *rec, int n,
+                                    T *out) {
+    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (q >= (long)nstate * n) return;
+    const int st = (int)(q / n), r = (int)(q % n);
+    const long o = g.at(rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
+    const T *v = states[st] + 3 * S;
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[3 * q + c] = v[c * S + o];
+}
+
 template <class T>
 void launch_adjoint(const AdjP<T> &A, const Work *work, int nwork, hipStream_t s) {
     if (nwork > 0) hipLaunchKernelGGL((k_adj_fused<T, 8>), dim3(nwork), dim3(64, 8), 0, s, A, work);
@@ -301,6 +319,14 @@ void launch_adjoint(const AdjP<T> &A, const Work *work, int nwork, hipStream_t s
 template <class T>
 void launch_adj_inject(T *const vh[3], const Grid<T> &g, const int *rec, int n, const T *res, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL((k_adj_inject<T>), dim3((n + 63) / 64), dim3(64), 0, s, vh[0], vh[1], vh[2], g, rec, n, res);
+}
+template <class T>
+void launch_rec_gather_states(const T *const *states, int nstate, size_t S, const Grid<T> &g, const int *rec, int n,
+                              T *out, hipStream_t s) {
+    const long m = (long)nstate * n;
+    if (m > 0)
+        hipLaunchKernelGGL((k_rec_gather_states<T>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, states, nstate,
+                           S, g, rec, n, out);
 }
 template <class T>
 void launch_rec_gather(T *const v[3], const Grid<T> &g, const int *rec, int n, T *out, hipStream_t s) {
@@ -343,7 +369,9 @@ void launch_adj_chain(const AdjChain<T> &c, hipStream_t s) {
     template void launch_adj_chain<T>(const AdjChain<T> &, hipStream_t);                              \
     template void launch_adjoint<T>(const AdjP<T> &, const Work *, int, hipStream_t);                 \
     template void launch_adj_inject<T>(T *const *, const Grid<T> &, const int *, int, const T *, hipStream_t); \
-    template void launch_rec_gather<T>(T *const *, const Grid<T> &, const int *, int, T *, hipStream_t);
+    template void launch_rec_gather<T>(T *const *, const Grid<T> &, const int *, int, T *, hipStream_t);         \
+    template void launch_rec_gather_states<T>(const T *const *, int, size_t, const Grid<T> &, const int *, int, T *, \
+                                              hipStream_t);
 INSTA(float)
 INSTA(double)
 

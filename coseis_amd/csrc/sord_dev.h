@@ -191,6 +191,10 @@ struct AdjP {
     T dt, h1, dth2, vol, rd[3];  // rd: 1 / (4 d_a)
     int lo[3], hi[3];          // real cells
     int nlo[3], nhi[3];        // real nodes
+    // residual added to the new adjoint velocity at receiver nodes: per work
+    // item Work.pad1 entries from Work.pad0 of rtile (j, k, l, receiver)
+    const int4 *rtile;
+    const T *res;              // this step's residuals (receiver x 3)
 };
 
 // the chain rule of the adjoint's material sums to cell rho, vp, vs

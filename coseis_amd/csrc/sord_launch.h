@@ -49,6 +49,7 @@ template <class T> void launch_adjoint(const AdjP<T> &A, const Work *work, int n
 template <class T> void launch_adj_chain(const AdjChain<T> &c, hipStream_t s);
 template <class T> void launch_adj_inject(T *const vh[3], const Grid<T> &g, const int *rec, int n, const T *res, hipStream_t s);
 template <class T> void launch_rec_gather(T *const v[3], const Grid<T> &g, const int *rec, int n, T *out, hipStream_t s);
+template <class T> void launch_rec_gather_states(const T *const *states, int nstate, size_t S, const Grid<T> &g, const int *rec, int n, T *out, hipStream_t s);
 template <class T> void launch_zero_stats(const StatZero<T> *z, int n, hipStream_t s);
 template <class T> void launch_fused_batch(const Scen<T> *sc, const Work *work, int nwork, int mask, bool st, hipStream_t s);
 }
