@@ -67,7 +67,7 @@ struct AdjSums {
 // ahead, and stored with the step's term added: a load consumed at once would
 // make the wave wait for every load issued before it, the next plane's
 // prefetch included.
-#define ADJ_NR 18
+#define ADJ_NR 15
 
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__restrict__ work) {
@@ -89,11 +89,9 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     auto fetch_node = [&](int l, T (&R)[ADJ_NR]) {
         const long o = g.at(jc, kc, min(l, lmax));
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
+        for (int i = 0; i < 3; i++)
             R[i] = A.uh[i][o], R[3 + i] = A.vh[i][o], R[6 + i] = A.u[i][o], R[9 + i] = A.v[i][o];
-            R[12 + i] = A.vn[i][o];
-        }
-        R[15] = A.mr[o], R[16] = A.gam[o], R[17] = A.gm[o];
+        R[12] = A.mr[o], R[13] = A.gam[o], R[14] = A.gm[o];
     };
     // lambda, mu, yy and the three gradient sums of cell l
     auto fetch_cell = [&](int l, T (&M)[6]) {
@@ -106,8 +104,8 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const T vbp = R[3 + i] + dt * R[i];
-            o9[i] = dt * R[15] * vbp;
-            o9[3 + i] = R[6 + i] + R[16] * R[9 + i];
+            o9[i] = dt * R[12] * vbp;
+            o9[3 + i] = R[6 + i] + R[13] * R[9 + i];
             o9[6 + i] = h1 * R[6 + i] + dth2 * R[9 + i];
         }
     };
@@ -194,13 +192,13 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     T EA[3][3], HA[3][3];                              // polynomials of the current cell (s_l = +1)
     T CeA[3][3], ChA[3][3], CeB[3][3], ChB[3][3];      // s_l = -1 polynomials (carried, new in turn)
     T RA[ADJ_NR], RB[ADJ_NR], mA[6], mB[6];
-    T ow[15];  // own node of the plane being updated: ub, vb, v, vn, 1/m, gamma, gradient sum
+    T ow[12];  // own node of the plane being updated: ub, vb, v, 1/m, gamma, gradient sum
     auto keep_own = [&](const T (&R)[ADJ_NR]) {
 #pragma unroll
         for (int i = 0; i < 6; i++) ow[i] = R[i];
 #pragma unroll
-        for (int i = 0; i < 3; i++) ow[6 + i] = R[9 + i], ow[9 + i] = R[12 + i];
-        ow[12] = R[15], ow[13] = R[16], ow[14] = R[17];
+        for (int i = 0; i < 3; i++) ow[6 + i] = R[9 + i];
+        ow[9] = R[12], ow[10] = R[13], ow[11] = R[14];
     };
     fetch_node(wk.lz0 - 1, RA);
     node_vals(RA, o9);
@@ -260,15 +258,24 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
 #pragma unroll
                     for (int c = 0; c < 3; c++) inj[c] = inj[c] + A.res[3 * r.w + c];
             }
+            // the 1/mass sum: (vb + dt ub) . (v_s+1 - v_s) over the steps,
+            // summed by parts so that a step needs only its own state's v:
+            // the new (vb + dt ub), which the next adjoint step takes as its
+            // input, times v_s, less the input's times v_s (none of the new
+            // at s = 1; the input of the first step times v_nt comes before
+            // the sweep, k_adj_gm_end)
             T gm = T(0);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const T vbp = ow[3 + c] + dt * ow[c];
-                A.uo[c][o] = ow[c] + fe[c] + h1 * fh[c];
-                A.vo[c][o] = (vbp + ow[13] * fe[c] + dth2 * fh[c]) + inj[c];
-                gm = gm + vbp * (ow[9 + c] - ow[6 + c]);
+                const T un = ow[c] + fe[c] + h1 * fh[c];
+                const T vn = (vbp + ow[10] * fe[c] + dth2 * fh[c]) + inj[c];
+                A.uo[c][o] = un;
+                A.vo[c][o] = vn;
+                const T vbo = A.gm_out ? vn + dt * un : T(0);
+                gm = gm + (vbo - vbp) * ow[6 + c];
             }
-            if (ow[12] != T(0)) A.gm[o] = ow[14] + gm / ow[12];
+            if (ow[9] != T(0)) A.gm[o] = ow[11] + gm / ow[9];
         }
         keep_own(Rc);
     };
@@ -276,6 +283,29 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
         plane(l, true, RA, mA, RB, mB, NB, NA, CeA, ChA, CeB, ChB);
         plane(l + 1, l + 1 <= wk.lz1, RB, mB, RA, mA, NA, NB, CeB, ChB, CeA, ChA);
     }
+}
+
+// the 1/mass sums' end term: the first adjoint step's input (vb: the
+// residual of the last step, ub zero) times v of the last forward state, at
+// the distinct receiver nodes (every other node's input is zero)
+template <class T>
+struct Ptr3 {
+    const T *p[3];
+};
+template <class T>
+__global__ void k_adj_gm_end(T *gm, const T *mr, Ptr3<T> vh, Ptr3<T> v, Grid<T> g, const int *node, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long o = g.at(node[3 * q], node[3 * q + 1], node[3 * q + 2]);
+    const T s = (vh.p[0][o] * v.p[0][o] + vh.p[1][o] * v.p[1][o]) + vh.p[2][o] * v.p[2][o];
+    if (mr[o] != T(0)) gm[o] = gm[o] + s / mr[o];
+}
+template <class T>
+void launch_adj_gm_end(T *gm, const T *mr, T *const vh[3], const T *const v[3], const Grid<T> &g, const int *node,
+                       int n, hipStream_t s) {
+    Ptr3<T> a, b;
+    for (int i = 0; i < 3; i++) a.p[i] = vh[i], b.p[i] = v[i];
+    if (n > 0) hipLaunchKernelGGL((k_adj_gm_end<T>), dim3((n + 63) / 64), dim3(64), 0, s, gm, mr, a, b, g, node, n);
 }
 
 // add w * r to the adjoint velocity at receiver nodes
@@ -369,6 +399,8 @@ void launch_adj_chain(const AdjChain<T> &c, hipStream_t s) {
     template void launch_adj_chain<T>(const AdjChain<T> &, hipStream_t);                              \
     template void launch_adjoint<T>(const AdjP<T> &, const Work *, int, hipStream_t);                 \
     template void launch_adj_inject<T>(T *const *, const Grid<T> &, const int *, int, const T *, hipStream_t); \
+    template void launch_adj_gm_end<T>(T *, const T *, T *const *, const T *const *, const Grid<T> &, const int *, int, \
+                                       hipStream_t);                                                   \
     template void launch_rec_gather<T>(T *const *, const Grid<T> &, const int *, int, T *, hipStream_t);         \
     template void launch_rec_gather_states<T>(const T *const *, int, size_t, const Grid<T> &, const int *, int, T *, \
                                               hipStream_t);

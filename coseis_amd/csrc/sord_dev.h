@@ -185,7 +185,7 @@ struct AdjP {
     const T *uh[3], *vh[3];   // adjoint state of step it+1
     T *uo[3], *vo[3];         // adjoint state of step it (out)
     const T *u[3], *v[3];     // forward state S_it
-    const T *vn[3];           // forward v of S_it+1
+    int gm_out;               // 0 on the last adjoint step (s = 1): no new (vb + dt ub) term
     const T *mr, *gam, *lam, *mu, *yy;
     T *gl, *gmu, *gy, *gm;    // material sums: cells lambda, mu, yy; nodes 1/mass
     T dt, h1, dth2, vol, rd[3];  // rd: 1 / (4 d_a)

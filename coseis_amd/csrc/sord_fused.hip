@@ -97,6 +97,41 @@ __device__ __forceinline__ T tmax(T a, T b) {
     return (b > a || b != b) ? b : a;
 }
 
+// the statistics maxima of a block (slots: 0 acceleration, 1 velocity, 2
+// displacement, 3 strain; 0 and 3 on a step that takes them now, 1 and 2 on
+// the step before): each wave's maxima through LDS to the first wave, then
+// one atomic per slot and block (the interior kernel).  Atomics of every
+// wave on the same four words queue in one L2 channel (16-wave blocks on
+// 200^3 nodes: the step with statistics 46 % slower than the step without).
+// Every wave of the block calls it.
+template <class T, int W>
+__device__ __forceinline__ void block_stat_max(const T (&m)[4], bool now, bool next, T *cur, T *nxt, int w,
+                                               int lane) {
+    __shared__ T sM[4][W];
+    if (!now && !next) return;  // uniform over the block
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool use = (q == 0 || q == 3) ? now : next;
+        if (use) {
+            const T x = wave_max(m[q]);
+            if (lane == 0) sM[q][w] = x;
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+        T r[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const bool use = (q == 0 || q == 3) ? now : next;
+            r[q] = use ? wave_max(lane < W ? sM[q][lane] : T(0)) : T(0);
+        }
+        if (lane == 0) {
+            if (now) atomic_max_bits<T>(cur + 0, r[0]), atomic_max_bits<T>(cur + 3, r[3]);
+            if (next) atomic_max_bits<T>(nxt + 1, r[1]), atomic_max_bits<T>(nxt + 2, r[2]);
+        }
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // shared pieces
@@ -444,16 +479,7 @@ __device__ __forceinline__ void interior_item(const Params<T> &P, const FusedC<T
             plane(l + 5, l + 5 <= wk.lz1, uB, vB, gB, mB, uA, vA, gA, mA, NA, NB, CB, CA);
         }
     }
-    if (stat_now) {
-        const T st_a = wave_max(stm[0]);
-        const T st_w = wave_max(stm[3]);
-        if (lane == 0) atomic_max_bits<T>(P.stat_cur + 0, st_a), atomic_max_bits<T>(P.stat_cur + 3, st_w);
-    }
-    if (stat_next) {
-        const T st_v = wave_max(stm[1]);
-        const T st_u = wave_max(stm[2]);
-        if (lane == 0) atomic_max_bits<T>(P.stat_nxt + 1, st_v), atomic_max_bits<T>(P.stat_nxt + 2, st_u);
-    }
+    if (ST) block_stat_max<T, W>(stm, stat_now, stat_next, P.stat_cur, P.stat_nxt, w, lane);
 }
 
 // one work item per block; the plain kernel gets one scenario in its
@@ -1047,6 +1073,13 @@ __device__ __forceinline__ void boundary_item(const Params<T> &P, const FusedC<T
         plane(l, true, fA, mA, fB, mB, SB, SA, CA, CB);
         plane(l + 1, l + 1 <= wk.lz1, fB, mB, fA, mA, SA, SB, CB, CA);
     }
+    // one atomic per wave and slot (four waves per block).  The block-level
+    // reduction of the interior kernel lost maxima here: double precision,
+    // the one-axis kernels of the layered model with free surface (tiles 8,
+    // 16, 4), 19 in 20 runs, the node with the largest acceleration missing
+    // from a step's maximum while its velocity was right; no cause was found
+    // in the kernel's source or its LDS layout, and this form lost none in
+    // 20 runs.
     if (stat_now) {
         st_a = wave_max(st_a);
         st_w = wave_max(st_w);

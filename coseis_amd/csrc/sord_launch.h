@@ -48,6 +48,7 @@ template <class T> FusedC<T> fused_consts(const Params<T> &P, const T sp[3], con
 template <class T> void launch_adjoint(const AdjP<T> &A, const Work *work, int nwork, hipStream_t s);
 template <class T> void launch_adj_chain(const AdjChain<T> &c, hipStream_t s);
 template <class T> void launch_adj_inject(T *const vh[3], const Grid<T> &g, const int *rec, int n, const T *res, hipStream_t s);
+template <class T> void launch_adj_gm_end(T *gm, const T *mr, T *const vh[3], const T *const v[3], const Grid<T> &g, const int *node, int n, hipStream_t s);
 template <class T> void launch_rec_gather(T *const v[3], const Grid<T> &g, const int *rec, int n, T *out, hipStream_t s);
 template <class T> void launch_rec_gather_states(const T *const *states, int nstate, size_t S, const Grid<T> &g, const int *rec, int n, T *out, hipStream_t s);
 template <class T> void launch_zero_stats(const StatZero<T> *z, int n, hipStream_t s);
