@@ -61,8 +61,8 @@ struct AdjSums {
     T F[3][3];  // F: J, K, X
 };
 
-// raw node loads of one plane: ub 0-2, vb 3-5, u 6-8, v 9-11, v of the next
-// state 12-14, 1/m 15, gamma 16, the node's gradient sum 17.  The gradient
+// raw node loads of one plane: ub 0-2, vb 3-5, u 6-8, v 9-11, 1/m 12, gamma
+// 13, the node's gradient sum 14.  The gradient
 // sums (node and cell) are loaded with the plane's other data, one plane
 // ahead, and stored with the step's term added: a load consumed at once would
 // make the wave wait for every load issued before it, the next plane's
@@ -344,7 +344,8 @@ __global__ void k_rec_gather_states(const T *const *states, int nstate, size_t S
 
 template <class T>
 void launch_adjoint(const AdjP<T> &A, const Work *work, int nwork, hipStream_t s) {
-    if (nwork > 0) hipLaunchKernelGGL((k_adj_fused<T, 8>), dim3(nwork), dim3(64, 8), 0, s, A, work);
+    constexpr int W = adj_waves<T>();
+    if (nwork > 0) hipLaunchKernelGGL((k_adj_fused<T, W>), dim3(nwork), dim3(64, W), 0, s, A, work);
 }
 template <class T>
 void launch_adj_inject(T *const vh[3], const Grid<T> &g, const int *rec, int n, const T *res, hipStream_t s) {

@@ -197,6 +197,13 @@ struct AdjP {
     const T *res;              // this step's residuals (receiver x 3)
 };
 
+// waves per block of the adjoint kernel (one node row each; the first and
+// last are halo rows).  12 in single precision fit its registers (168
+// VGPRs, two spilled: three waves per SIMD, a sixth less halo) but measured
+// no faster (200^3 backward 507-523 against 513-519 ms).
+template <class T>
+constexpr int adj_waves() { return 8; }
+
 // the chain rule of the adjoint's material sums to cell rho, vp, vs
 template <class T>
 struct AdjChain {

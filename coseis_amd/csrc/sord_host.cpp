@@ -307,14 +307,24 @@ struct Solver {
     bool no_alloc = false;
     unsigned char *sscratch = nullptr;  // this device's static scratch (static_scratch)
 
+    // The k-th array starts (k mod 13) x 4 KB past its allocation: the arrays
+    // a kernel streams side by side (allocations all begin on the same large
+    // boundary) then sit at different offsets of the memory's interleave
+    // (layered 321 x 321 x 161 model: 0.565 against 0.554 ms per step; a
+    // 256 B, 2 KB or 64 KB unit gives the same).
+    int dalloc_n = 0;
     T *dalloc(size_t n) {
         if (no_alloc) fail(SORD_ESTATE, "sord_step_static: this step would allocate device memory");
         void *p = nullptr;
-        HIPCHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
-        HIPCHECK(hipMemsetAsync(p, 0, (n ? n : 1) * sizeof(T), stream));
+        const size_t shift = (size_t)(dalloc_n++ % 13) * 4096, bytes = (n ? n : 1) * sizeof(T) + shift;
+        HIPCHECK(hipMalloc(&p, bytes));
+        HIPCHECK(hipMemsetAsync(p, 0, bytes, stream));
         dallocs.push_back(p);
-        return (T *)p;
+        T *a = (T *)((char *)p + shift);
+        dbase.push_back({a, p});
+        return a;
     }
+    std::vector<std::pair<const T *, void *>> dbase;  // array -> its allocation (dalloc)
 
     // ---------------- read_parameters: sord/input_output.f90:21-98 ----------------
     void parse(const char *text) {
