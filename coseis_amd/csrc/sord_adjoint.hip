@@ -61,6 +61,22 @@ struct AdjSums {
     T F[3][3];  // F: J, K, X
 };
 
+// An array element as the array's base (scalar registers) plus a 32-bit byte
+// offset, one per node and plane shared by every array: the load or store
+// adds the two itself, where a 64-bit element index costs a 64-bit add per
+// access (76 of the plane loop's 818 VALU instructions).  One array of the
+// padded mesh stays below 4 GB (adj_setup).
+template <class T>
+__device__ __forceinline__ T ldo(const T *base, unsigned ob) {
+    using G = const __attribute__((address_space(1))) char *;
+    return *(const __attribute__((address_space(1))) T *)((G)(const char *)base + ob);
+}
+template <class T>
+__device__ __forceinline__ void sto(T *base, unsigned ob, T v) {
+    using G = __attribute__((address_space(1))) char *;
+    *(__attribute__((address_space(1))) T *)((G)(char *)base + ob) = v;
+}
+
 // raw node loads of one plane: ub 0-2, vb 3-5, u 6-8, v 9-11, 1/m 12, gamma
 // 13, the node's gradient sum 14.  The gradient
 // sums (node and cell) are loaded with the plane's other data, one plane
@@ -86,18 +102,19 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     const T dt = A.dt, h1 = A.h1, dth2 = A.dth2, V = A.vol;
     const T ax = V * A.rd[0], ay = V * A.rd[1], az = V * A.rd[2];
 
+    constexpr unsigned TB = sizeof(T);
     auto fetch_node = [&](int l, T (&R)[ADJ_NR]) {
-        const long o = g.at(jc, kc, min(l, lmax));
+        const unsigned ob = (unsigned)g.at(jc, kc, min(l, lmax)) * TB;
 #pragma unroll
         for (int i = 0; i < 3; i++)
-            R[i] = A.uh[i][o], R[3 + i] = A.vh[i][o], R[6 + i] = A.u[i][o], R[9 + i] = A.v[i][o];
-        R[12] = A.mr[o], R[13] = A.gam[o], R[14] = A.gm[o];
+            R[i] = ldo(A.uh[i], ob), R[3 + i] = ldo(A.vh[i], ob), R[6 + i] = ldo(A.u[i], ob), R[9 + i] = ldo(A.v[i], ob);
+        R[12] = ldo(A.mr, ob), R[13] = ldo(A.gam, ob), R[14] = ldo((const T *)A.gm, ob);
     };
     // lambda, mu, yy and the three gradient sums of cell l
     auto fetch_cell = [&](int l, T (&M)[6]) {
-        const long o = g.at(jc, kc, min(l, lmax));
-        M[0] = A.lam[o], M[1] = A.mu[o], M[2] = A.yy[o];
-        M[3] = A.gl[o], M[4] = A.gmu[o], M[5] = A.gy[o];
+        const unsigned ob = (unsigned)g.at(jc, kc, min(l, lmax)) * TB;
+        M[0] = ldo(A.lam, ob), M[1] = ldo(A.mu, ob), M[2] = ldo(A.yy, ob);
+        M[3] = ldo((const T *)A.gl, ob), M[4] = ldo((const T *)A.gmu, ob), M[5] = ldo((const T *)A.gy, ob);
     };
     // exchanged: r = dt m (vb + dt ub) 0-2, w = u + gamma v 3-5, F = h1 u + dth2 v 6-8
     auto node_vals = [&](const T (&R)[ADJ_NR], T (&o9)[9]) {
@@ -174,16 +191,16 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
             Ch[c][0] = h[1], Ch[c][1] = h[0], Ch[c][2] = -h[2] + h[3];
         }
         if (acc && real) {
-            const long o = g.at(j, k, l);
+            const unsigned ob = (unsigned)g.at(j, k, l) * TB;
             const T tr = G[0][0] + G[1][1] + G[2][2];
             T sm = T(0);
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int c = 0; c < 3; c++) sm = sm + Gr[a][c] * (G[a][c] + G[c][a]);
-            A.gl[o] = M[3] + -V * tr * trr;
-            A.gmu[o] = M[4] + -V * sm;
-            A.gy[o] = M[5] + -sy;
+            sto(A.gl, ob, M[3] + -V * tr * trr);
+            sto(A.gmu, ob, M[4] + -V * sm);
+            sto(A.gy, ob, M[5] + -sy);
         }
     };
 
@@ -249,7 +266,7 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
             fh[c] = (q0 + qj) + adj_prev(q0 - qj);
         }
         if (own && act) {
-            const long o = g.at(j, k, l);
+            const unsigned ob = (unsigned)g.at(j, k, l) * TB;
             // the residual at receiver nodes of this tile (uniform loop)
             T inj[3] = {T(0), T(0), T(0)};
             for (int e = wk.pad0; e < wk.pad0 + wk.pad1; e++) {
@@ -270,12 +287,12 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
                 const T vbp = ow[3 + c] + dt * ow[c];
                 const T un = ow[c] + fe[c] + h1 * fh[c];
                 const T vn = (vbp + ow[10] * fe[c] + dth2 * fh[c]) + inj[c];
-                A.uo[c][o] = un;
-                A.vo[c][o] = vn;
+                sto(A.uo[c], ob, un);
+                sto(A.vo[c], ob, vn);
                 const T vbo = A.gm_out ? vn + dt * un : T(0);
                 gm = gm + (vbo - vbp) * ow[6 + c];
             }
-            if (ow[9] != T(0)) A.gm[o] = ow[11] + gm / ow[9];
+            if (ow[9] != T(0)) sto(A.gm, ob, ow[11] + gm / ow[9]);
         }
         keep_own(Rc);
     };
