@@ -1,9 +1,9 @@
 // Device-side state and parameters of the SORD solver step.
 //
 // Layout in HBM: every volume field is a 3-D array over the padded local
-// mesh nm = nn + 2*nhalo (sord/setup.f90:56-57), j fastest, stored with a row
-// pitch P0 (a multiple of 128 bytes) and a base offset chosen so that the first
-// interior node of every row starts on a 128-byte boundary.  Vectors and
+// mesh nm = nn + 2*nhalo (sord/setup.f90:56-57), j fastest, with no row
+// padding (row pitch P0 = nm[0]) and a base offset that puts the array's first
+// interior node on a 128-byte boundary.  Vectors and
 // tensors are structure-of-arrays: one such 3-D array per component.
 // Indices used in kernels are the reference's 1-based local indices.
 #pragma once
