@@ -198,11 +198,12 @@ struct AdjP {
 };
 
 // waves per block of the adjoint kernel (one node row each; the first and
-// last are halo rows).  12 in single precision fit its registers (168
-// VGPRs, two spilled: three waves per SIMD, a sixth less halo) but measured
-// no faster (200^3 backward 507-523 against 513-519 ms).
+// last are halo rows): 12 in single precision, which fit its registers (168
+// VGPRs, two spilled: three waves per SIMD, a sixth less halo; 200^3 adjoint
+// step 305 -> 287 us by rocprofv3), 8 in double precision (its registers
+// spill at any count)
 template <class T>
-constexpr int adj_waves() { return 8; }
+constexpr int adj_waves() { return sizeof(T) == 4 ? 12 : 8; }
 
 // the chain rule of the adjoint's material sums to cell rho, vp, vs
 template <class T>

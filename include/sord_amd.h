@@ -208,8 +208,8 @@ void sord_batch_destroy(void *batch);
 int sord_adjoint_setup(void *handle, int nrec, const int32_t *rec_nodes, int checkpoint_every);
 /* Before sord_adjoint_setup: how many of the forward run's last states to
  * keep resident in device memory, so that the backward sweep reads them
- * instead of recomputing their segments: -1 as many as fit in 95 % of the
- * free memory (less 2 GB), 0 none (the default: checkpoints only), n > 0 at
+ * instead of recomputing their segments: -1 as many as fit in 98 % of the
+ * free memory (less 1 GB), 0 none (the default: checkpoints only), n > 0 at
  * most n.  Rounded to whole segments.  No counterpart in the reference. */
 int sord_adjoint_set_resident(void *handle, int64_t states);
 
