@@ -97,7 +97,10 @@ __global__ __launch_bounds__(64 * W) void k_adj_fused(AdjP<T> A, const Work *__r
     const bool colreal = lane <= 62 && w <= W - 2 && j <= wk.je && k <= wk.ke && j >= A.lo[0] && j <= A.hi[0] &&
                          k >= A.lo[1] && k <= A.hi[1];
     const Grid<T> &g = A.g;
-    const int jc = min(max(j, 1), g.nm[0]), kc = min(max(k, 1), g.nm[1]);
+    // load coordinates: lanes past the column after the tile's last and waves
+    // past the row after its last feed nothing, and load that column or row
+    // again (lines already fetched) instead of the next tile's
+    const int jc = min(max(j, 1), min(g.nm[0], wk.je + 1)), kc = min(max(k, 1), min(g.nm[1], wk.ke + 1));
     const int lmax = g.nm[2];
     const T dt = A.dt, h1 = A.h1, dth2 = A.dth2, V = A.vol;
     const T ax = V * A.rd[0], ay = V * A.rd[1], az = V * A.rd[2];
